@@ -1,0 +1,213 @@
+"""Write the golden fixtures under tests/golden/ from the reference itself.
+
+Runs oracle/_ref/ref_driver (the reference's Fortran modules compiled by
+`make -C oracle ref`) on decks written by refdeck.py and stores inputs and outputs as .npz.
+Only this script touches the reference build; the tests read the .npz files.
+
+    python tests/golden/make_golden.py            # all fixtures
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import refdeck  # noqa: E402
+import refrun  # noqa: E402
+from refrun import pack, Reader  # noqa: E402
+
+SYSTEMS = {14: 3, 38: 4, 66: 5}
+
+
+def box(n, rho=0.0337737278):
+    return (n / rho) ** (1.0 / 3.0)
+
+
+def setup_fixture(n, norb):
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    el = box(n)
+    rr = np.concatenate([[0.0], np.linspace(1e-4, 0.5 * el, 97), [0.5 * el - 1e-9],
+                         np.random.default_rng(n).uniform(0.0, 0.5 * el, 30)])
+    seeds = np.array([1, 7, 11, 17, 281474976710655, 123456789012345], dtype=np.int64)
+    ns = 64
+    data, _ = refrun.run("setup", deck, pack([norb], [len(rr)], rr, [len(seeds)], seeds, [ns]))
+    r = Reader(data)
+    el_, eunit, etrial = r.f(3)
+    nk = r.i()
+    numsh = r.i(norb)
+    ak = r.f(3 * nk).reshape(nk, 3)
+    ak2 = r.f(nk)
+    rad = np.array([r.f(9) for _ in rr])
+    s1 = np.zeros((len(seeds), ns), np.int64)
+    u1 = np.zeros((len(seeds), ns))
+    s2 = np.zeros((len(seeds), ns), np.int64)
+    u2 = np.zeros((len(seeds), ns))
+    for j in range(len(seeds)):
+        for i in range(ns):
+            s1[j, i] = r.i()
+            u1[j, i] = r.f()
+            s2[j, i] = r.i()
+            u2[j, i] = r.f()
+    g7 = np.zeros((len(seeds), 7))
+    gstate = np.zeros(len(seeds), np.int64)
+    for j in range(len(seeds)):
+        g7[j] = r.f(7)
+        gstate[j] = r.i()
+    assert r.done()
+    np.savez(os.path.join(HERE, f"setup_N{n}.npz"), npart=n, norb=norb, el=el_, eunit=eunit,
+             etrial=etrial, nk=nk, numsh=numsh, ak=ak, ak2=ak2, r=rr, radial=rad[:, 0:3],
+             beta=rad[:, 3:6], derbeta=rad[:, 6:8], vpot=rad[:, 8], seeds=seeds,
+             ran1_state=s1, ran1_u=u1, ran2_state=s2, ran2_u=u2, gauss7=g7,
+             gauss7_state=gstate, deck=deck)
+    return el_
+
+
+def random_configs(n, nc, seed):
+    el = box(n)
+    rng = np.random.default_rng(seed)
+    return el * (rng.random((nc, n, 3)) - 0.5)
+
+
+def hpsi_fixture(n, norb, configs, tag):
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    nc = len(configs)
+    data, _ = refrun.run("hpsi", deck, pack([nc], configs))
+    r = Reader(data)
+    out = {k: [] for k in ("u", "d2u", "v", "phil", "d2phi", "isdet", "du", "dphi", "psil",
+                           "d2psi", "is", "psigl", "d2psig", "dpsi", "dpsig")}
+    for _ in range(nc):
+        u, d2u, v, phil, d2phi, isd = r.f(6)
+        du = r.f(3 * n).reshape(n, 3)
+        dphi = r.f(3 * n).reshape(n, 3)
+        psil, d2psi, v2, is_, psigl, d2psig = r.f(6)
+        dpsi = r.f(3 * n).reshape(n, 3)
+        dpsig = r.f(3 * n).reshape(n, 3)
+        for k, val in (("u", u), ("d2u", d2u), ("v", v), ("phil", phil), ("d2phi", d2phi),
+                       ("isdet", isd), ("du", du), ("dphi", dphi), ("psil", psil),
+                       ("d2psi", d2psi), ("is", is_), ("psigl", psigl), ("d2psig", d2psig),
+                       ("dpsi", dpsi), ("dpsig", dpsig)):
+            out[k].append(val)
+    assert r.done()
+    np.savez(os.path.join(HERE, f"hpsi_{tag}.npz"), npart=n, norb=norb, x=configs, deck=deck,
+             **{k: np.array(v) for k, v in out.items()})
+
+
+def ratio_fixture(n, norb, configs, nm, seed, tag):
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    rng = np.random.default_rng(seed)
+    nc = len(configs)
+    ip = rng.integers(1, n + 1, size=(nc, nm))
+    step = rng.normal(0.0, 0.3, size=(nc, nm, 3))
+    xnew = configs[np.arange(nc)[:, None], ip - 1] + step
+    parts = [pack([nc], [nm])]
+    for k in range(nc):
+        parts.append(pack(configs[k]))
+        for m in range(nm):
+            parts.append(pack([ip[k, m]], xnew[k, m]))
+    data, _ = refrun.run("ratio", deck, b"".join(parts))
+    r = Reader(data)
+    rat = r.f(nc * nm).reshape(nc, nm)
+    assert r.done()
+    np.savez(os.path.join(HERE, f"ratio_{tag}.npz"), npart=n, norb=norb, x=configs, ip=ip,
+             xnew=xnew, ratio=rat, deck=deck)
+
+
+def der_fixture(n, norb, configs, tag):
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    nc = len(configs)
+    data, _ = refrun.run("der", deck, pack([nc], configs))
+    r = Reader(data)
+    npar = r.i()
+    par = r.f(npar)
+    dp = r.f(nc * npar).reshape(nc, npar)
+    assert r.done()
+    np.savez(os.path.join(HERE, f"der_{tag}.npz"), npart=n, norb=norb, x=configs, par=par,
+             dlogpsi=dp, deck=deck)
+
+
+def read_dump(r, n):
+    nw = r.i()
+    irn = np.zeros(nw, np.int64)
+    sc = np.zeros((nw, 6))
+    x = np.zeros((nw, n, 3))
+    dpsi = np.zeros((nw, n, 3))
+    for k in range(nw):
+        irn[k] = r.i()
+        sc[k] = r.f(6)
+        x[k] = r.f(3 * n).reshape(n, 3)
+        dpsi[k] = r.f(3 * n).reshape(n, 3)
+    return irn, sc, x, dpsi
+
+
+def dmc_fixture(tag, dumps, keep_x=True, init=None, **deckargs):
+    deck, p = refdeck.make_deck(**deckargs)
+    n = p["npart"]
+    parts = [pack([len(dumps)], np.asarray(dumps, np.int64))]
+    mode = "dmc"
+    if init is not None:
+        mode = "dmcfile"
+        x0, irn0 = init
+        parts.append(pack([len(x0)]))
+        for k in range(len(x0)):
+            parts.append(pack(x0[k], [irn0[k]]))
+    data, _ = refrun.run(mode, deck, b"".join(parts))
+    r = Reader(data)
+    nsteps = r.i()
+    nstep = p["nstep"]
+    out = {}
+    pop = np.zeros((nsteps, 2), np.int64)
+    est = []
+    if 0 in dumps:
+        out["d0"] = read_dump(r, n)
+    for jj in range(1, nsteps + 1):
+        pop[jj - 1] = r.i(2)
+        if jj in dumps:
+            out[f"d{jj}"] = read_dump(r, n)
+        if jj % nstep == 0:
+            est.append(r.f(28).reshape(14, 2))
+    assert r.done()
+    arrs = dict(npart=n, norb=p["norb"], pop=pop, est=np.array(est), deck=deck,
+                dumps=np.asarray(dumps, np.int64), nwalk=p["nwalk"], nav=p["nav"],
+                nstep=nstep, dt=p["dt"], seed=p["seed"])
+    for key, (irn, sc, x, dpsi) in out.items():
+        arrs[f"{key}_irn"] = irn
+        arrs[f"{key}_sc"] = sc
+        if keep_x:
+            arrs[f"{key}_x"] = x
+            arrs[f"{key}_dpsi"] = dpsi
+    if init is not None:
+        arrs["init_x"], arrs["init_irn"] = init
+    np.savez_compressed(os.path.join(HERE, f"dmc_{tag}.npz"), **arrs)
+    return out
+
+
+def main():
+    for n, norb in SYSTEMS.items():
+        setup_fixture(n, norb)
+    # hpsi on uniform configurations (all three systems) and with 8 shells at N=66
+    hpsi_fixture(14, 3, random_configs(14, 32, 1014), "N14")
+    hpsi_fixture(38, 4, random_configs(38, 12, 1038), "N38")
+    hpsi_fixture(66, 5, random_configs(66, 12, 1066), "N66")
+    hpsi_fixture(66, 8, random_configs(66, 4, 2066), "N66_norb8")
+    # single-particle ratios (bcsone * getphione through move1)
+    ratio_fixture(14, 3, random_configs(14, 8, 3014), 6, 4014, "N14")
+    ratio_fixture(66, 5, random_configs(66, 4, 3066), 6, 4066, "N66")
+    # d log psi / d alpha_s (getderbcs)
+    der_fixture(14, 3, random_configs(14, 8, 5014), "N14")
+    der_fixture(66, 8, random_configs(66, 4, 5066), "N66_norb8")
+    # short DMC trajectories with walker dumps (multiset parity of branching)
+    dmc_fixture("N14_short", [0, 1, 2, 5, 10, 20], npart=14, norb=3, nwalk=200, nav=2,
+                nstep=10, dt=0.002, seed=1)
+    dmc_fixture("N38_short", [0, 1, 3], npart=38, norb=4, nwalk=48, nav=1, nstep=3,
+                dt=0.001, seed=5)
+    dmc_fixture("N66_short", [0, 1, 2, 4], npart=66, norb=5, nwalk=32, nav=2, nstep=2,
+                dt=0.001, seed=7)
+    # the N=14 production configuration: 1000 walkers, 500 steps, population trace,
+    # block estimators and the final (irn, weight) multiset
+    dmc_fixture("N14_cfg1", [500], keep_x=False, npart=14, norb=3, nwalk=1000, nav=5,
+                nstep=100, dt=0.002, seed=1)
+
+
+if __name__ == "__main__":
+    main()
