@@ -1,0 +1,237 @@
+// All-particle (reference-replay) kernels: hpsi of a walker in one 64-lane workgroup, the
+// fused DMC step (Gaussian drift-diffusion + hpsi + weight + node test + estimators +
+// branch multiplicity), single-particle ratios, and the parameter adjoint.
+//
+// Work split inside a walker: the 64 lanes form an 8 x 8 grid; lane (r, c) owns up
+// particles i = r + 8a and down particles j = c + 8b (a, b < TR = ceil(n/8)), i.e. a
+// TR x TR tile of the n x n pairing matrix S(i, j) = phi(r_up_i - r_dn_j).  The tile stays
+// in registers through the Gauss-Jordan inversion; only the pivot row and the pivot column
+// travel through LDS.  Plane-wave phases exp(i q.r) of every particle are built from three
+// sincos per particle and staged in LDS.
+#include "dmc_wave.h"
+
+namespace dmc {
+
+// ------------------------------------------------------------------------------------
+// kernels
+
+template <int NUP>
+__global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict__ x, int nw,
+                                             double *psil, int *is, double *dpsi,
+                                             double *d2psi, double *v) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= nw) return;
+    constexpr int M = 6 * NUP;
+    load_q<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
+    __syncthreads();
+    hpsi_block<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) dpsi[size_t(w) * M + t] = sm[L.dp + t];
+    if (tid == 0) {
+        psil[w] = sm[L.scal];
+        d2psi[w] = sm[L.scal + 1];
+        v[w] = sm[L.scal + 2];
+        is[w] = int(sm[L.scal + 3]);
+    }
+}
+
+// The fused DMC step of the reference (step.f90:56-76,127-160 with idmc = 1) for walker w
+// of state A, written to the same slot of state B, followed by the branch draw
+// (step.f90:302-305): B.irn holds the state after that draw and T.count[w] = iwt.
+template <int NUP>
+__global__ __launch_bounds__(64) void k_step(DevSys S, DevState A, DevState B, DevStep T,
+                                             int opcomp) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= T.nt[0]) return;
+    constexpr int M = 6 * NUP;                // 3 * npart coordinates
+    constexpr int ND = 2 * ((M + 1) / 2);     // uniforms drawn by gaussian(M)
+    load_q<NUP>(S, sm, L, tid);
+    const uint64_t s0 = A.irn[w];
+    const double sig = S.sigma;
+    const double dfac = S.sig2 * S.sqrtm;
+    const double *xo = A.x + size_t(w) * M, *go = A.dpsi + size_t(w) * M;
+    // Box-Muller pairs (ran.f90:85-102); draw t uses the jump-ahead state after t+1 draws
+    for (int m = tid; 2 * m < M; m += 64) {
+        const int t0 = 2 * m, t1 = 2 * m + 1;
+        const uint64_t a0 = (S.ja1[t0] * s0 + S.jc1[t0]) & kMask48;
+        const uint64_t a1 = (S.ja1[t1] * s0 + S.jc1[t1]) & kMask48;
+        const double u1 = double(a0 | 1ull) * 0x1p-48, u2 = double(a1 | 1ull) * 0x1p-48;
+        double sn, cs;
+        sincos(2.0 * kPi * u1, &sn, &cs);
+        const double rad = sqrt(-2.0 * log(u2));
+        // odd count: the last value uses cos (ran.f90:95-102)
+        const double g0 = (t1 == M ? cs : sn) * rad, g1 = cs * rad;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int t = t0 + e;
+            if (t < M) {
+                double drift = go[t] * dfac;
+                if (S.cutdrift) drift = fmin(S.dcut, fmax(-S.dcut, drift));
+                const double x1 = wrapd(xo[t], S);
+                sm[L.xs + t] = wrapd(x1 + (drift + sig * (e ? g1 : g0)), S);
+            }
+        }
+    }
+    __syncthreads();
+    hpsi_block<NUP>(S, sm, L, tid);
+    double *xn = B.x + size_t(w) * M, *gn = B.dpsi + size_t(w) * M;
+    for (int t = tid; t < M; t += 64) {
+        xn[t] = sm[L.xs + t];
+        gn[t] = sm[L.dp + t];
+    }
+    double *est = T.est + size_t(w) * 42;
+    if (tid < 42) est[tid] = 0.0;
+    __syncthreads();
+    if (tid == 0) {
+        const double psil2 = sm[L.scal], d2psi2 = sm[L.scal + 1], v2 = sm[L.scal + 2];
+        const int is2 = int(sm[L.scal + 3]);
+        const double en = -S.hbar * d2psi2 + v2;
+        const double ego = -S.hbar * A.d2psi[w] + A.v[w];
+        double wt = exp(-(0.5 * (ego + en) - S.etrial) * S.dt);
+        double weight = fmin(S.wtcut, wt);
+        const bool cross = is2 != A.is[w];
+        if (cross && S.fn) {
+            weight = 0.0;
+            wt = 0.0;
+        }
+        est[12] = cross ? 1.0 : 0.0;  // node crossing (estimator 5): val, val^2, wt
+        est[13] = cross ? 1.0 : 0.0;
+        est[14] = 1.0;
+        const double wt0 = A.wt0[w];
+        const double wtg = wt * double(is2) * wt0;
+        if (opcomp) {
+            // estimators 1-4 and 6-14 of step.f90:142-156 (addval: val*wt, val^2*wt, wt)
+            const double nf = S.nfac, sg = copysign(1.0, wtg);
+            auto put = [&](int e, double val, double wgt) {
+                est[3 * e] = val * wgt;
+                est[3 * e + 1] = val * val * wgt;
+                est[3 * e + 2] = wgt;
+            };
+            put(0, en * nf, wtg);
+            put(1, (en - v2) * nf, wtg);
+            put(2, v2 * nf, wtg);
+            put(3, 0.0, wtg);
+            put(5, 0.5 * (ego + en) * nf, 1.0);
+            put(6, 1.0, 1.0);
+            put(7, is2 * exp(psil2), 1.0);
+            put(8, exp(psil2), 1.0);
+            put(9, double(is2), 1.0);
+            put(10, wt, 1.0);
+            put(11, wtg, 1.0);
+            put(12, sg, 1.0);
+            put(13, sg, wtg);
+        }
+        // new RNG state after gaussian(M), then the branch draw
+        uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;
+        s1 = (s1 * kM1 + kA1) & kMask48;
+        const double ub = double(s1 | 1ull) * 0x1p-48;
+        const int iwt = int(weight + ub);
+        T.count[w] = iwt;
+        if (iwt >= 0 && iwt < 500)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.hist[iwt]), 1ull);
+        B.irn[w] = s1;
+        B.psil[w] = psil2;
+        B.d2psi[w] = d2psi2;
+        B.v[w] = v2;
+        B.is[w] = is2;
+        B.wt0[w] = wt0;
+        B.weight[w] = weight;
+    }
+}
+
+// single-particle ratio psi(x')/psi(x) (move1, wavefunction.f90:393-404): determinant
+// ratio from the inverse (bcsone, psislaterbcs.f90:296-376) times the Jastrow ratio
+template <int NUP>
+__global__ __launch_bounds__(64) void k_ratio(DevSys S, const double *__restrict__ x,
+                                              const int *ip, const double *xnew, int nw,
+                                              double *ratio) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    constexpr int SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= nw) return;
+    constexpr int M = 6 * NUP;
+    load_q<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
+    double *xn = sm + L.xn;
+    if (tid < 3) xn[tid] = xnew[3 * w + tid];
+    __syncthreads();
+    const Tile<NUP> T(tid);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
+    const int i = ip[w];
+    const double2 *ph = reinterpret_cast<const double2 *>(sm + L.work);
+    double2 *phn = reinterpret_cast<double2 *>(sm + L.xn + 4);
+    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
+    if (tid == 0) phases_of(S, qm, xn, phn, 1);
+    __syncthreads();
+    const double *qc = sm + L.qc, *sinv = sm + L.sinv, *xs = sm + L.xs;
+    double part = 0.0;
+    if (tid < NUP) {
+        const int j = tid;
+        if (i < NUP) {
+            const double f = phi_pair(S, qc, phn, ph + (NUP + j) * S.nq, xn, xs + 3 * (NUP + j));
+            part = sinv[j * SP + i] * f;
+        } else {
+            const double f = phi_pair(S, qc, ph + j * S.nq, phn, xs + 3 * j, xn);
+            part = sinv[(i - NUP) * SP + j] * f;
+        }
+    }
+    const double rdet = wave_sum(part);
+    const double lj = jastrow_logratio<NUP>(S, xs, i, xn, tid);
+    if (tid == 0) ratio[w] = D.singular ? 0.0 : exp(lj) * rdet;
+    (void)NP;
+}
+
+// ------------------------------------------------------------------------------------
+// launchers
+
+bool nup_supported(int nup) { return nup == 1 || nup == 7 || nup == 19 || nup == 33; }
+
+#define DMC_DISPATCH(NUPV, CALL)          \
+    switch (NUPV) {                       \
+        case 1: { constexpr int N_ = 1; CALL; } break;   \
+        case 7: { constexpr int N_ = 7; CALL; } break;   \
+        case 19: { constexpr int N_ = 19; CALL; } break; \
+        case 33: { constexpr int N_ = 33; CALL; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
+int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
+                double *dpsi, double *d2psi, double *v, hipStream_t st) {
+    if (nw <= 0) return hipSuccess;
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_eval<N_>, dim3(nw), dim3(64), lds_bytes<N_>(S.nq), st,
+                                         S, x, nw, psil, is, dpsi, d2psi, v));
+    return hipGetLastError();
+}
+
+int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const double *xnew,
+                 int nw, double *ratio, hipStream_t st) {
+    if (nw <= 0) return hipSuccess;
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_ratio<N_>, dim3(nw), dim3(64), lds_bytes<N_>(S.nq), st,
+                                         S, x, ip, xnew, nw, ratio));
+    return hipGetLastError();
+}
+
+int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                const DevStep &T, int opcomp, int grid, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_step<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq), st,
+                                         S, A, B, T, opcomp));
+    return hipGetLastError();
+}
+
+size_t allpart_lds_bytes(int nup, int nq) {
+    switch (nup) {
+        case 1: return lds_bytes<1>(nq);
+        case 7: return lds_bytes<7>(nq);
+        case 19: return lds_bytes<19>(nq);
+        case 33: return lds_bytes<33>(nq);
+    }
+    return 0;
+}
+
+}  // namespace dmc

@@ -1,0 +1,575 @@
+// The C-ABI of include/dmc_gpu.h: context, device memory, uploads and the per-step
+// orchestration (fused step kernel -> estimator reduction -> branch compaction ->
+// population control).
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dmc_internal.h"
+#include "dmc_kernels.h"
+
+namespace dmc {
+
+__global__ void k_init_weights(DevState A, int nw) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    A.wt0[w] = double(A.is[w]);
+    A.weight[w] = 1.0;
+}
+
+}  // namespace dmc
+
+using namespace dmc;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(DMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(expr)                                                                          \
+    do {                                                                                         \
+        int e_ = (expr);                                                                         \
+        if (e_ != 0)                                                                             \
+            return fail(DMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(hipError_t(e_))); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1));
+}
+}  // namespace
+
+struct dmc_ctx {
+    System sys;
+    DevSys ds{};
+    hipStream_t st = nullptr;
+    int capacity = 0, nw = 0;
+    std::vector<void *> allocs;
+    DevState A{}, B{};
+    DevStep T{};
+    double *est_acc = nullptr;
+    int *h_nt = nullptr;  // pinned
+    bool timing = false;
+    double time_ms = 0.0;
+    int64_t launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int64_t acc_moves = 0, prop_moves = 0;
+
+    template <class T>
+    int alloc(T **p, size_t n) {
+        hipError_t e = dalloc(p, n);
+        if (e != hipSuccess) return fail(DMC_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(*p);
+        return 0;
+    }
+    int alloc_state(DevState &S) {
+        const size_t m = size_t(capacity) * 3 * sys.npart;
+        if (alloc(&S.x, m) || alloc(&S.dpsi, m) || alloc(&S.psil, capacity) ||
+            alloc(&S.d2psi, capacity) || alloc(&S.v, capacity) || alloc(&S.weight, capacity) ||
+            alloc(&S.wt0, capacity) || alloc(&S.is, capacity) || alloc(&S.irn, capacity))
+            return DMC_ERR_HIP;
+        return 0;
+    }
+    int ensure_scratch(size_t bytes) {
+        if (bytes <= scratch_bytes) return 0;
+        if (scratch) hipFree(scratch);
+        scratch = nullptr;
+        scratch_bytes = 0;
+        hipError_t e = hipMalloc(&scratch, bytes);
+        if (e != hipSuccess) return fail(DMC_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        scratch_bytes = bytes;
+        return 0;
+    }
+};
+
+namespace {
+
+int upload_tables(dmc_ctx *c) {
+    const System &s = c->sys;
+    const int n1 = s.ntab + 1;
+    std::vector<double4> jt(n1), bt(n1);
+    std::vector<float4> jt32(n1), bt32(n1);
+    for (int i = 0; i < n1; ++i) {
+        jt[i] = make_double4(s.vt[i], s.ut[i], s.dut[i], s.d2ut[i]);
+        bt[i] = make_double4(s.bt[i], s.dbt[i], s.d2bt[i], 0.0);
+        jt32[i] = make_float4(float(s.vt[i]), float(s.ut[i]), float(s.dut[i]), float(s.d2ut[i]));
+        bt32[i] = make_float4(float(s.bt[i]), float(s.dbt[i]), float(s.d2bt[i]), 0.0f);
+    }
+    HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.jt), jt.data(), sizeof(double4) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.bt), bt.data(), sizeof(double4) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.jt32), jt32.data(), sizeof(float4) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.bt32), bt32.data(), sizeof(float4) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int upload_q(dmc_ctx *c) {
+    const System &s = c->sys;
+    const int nq = s.nk - 1;
+    std::vector<int4> qm(nq > 0 ? nq : 1);
+    std::vector<double> wq(nq > 0 ? nq : 1);
+    for (int q = 0; q < nq; ++q) {
+        const auto &v = s.ik[q + 1];
+        qm[q] = make_int4(v[0], v[1], v[2], s.kshell[q + 1]);
+        wq[q] = 2.0 * s.vorb[s.kshell[q + 1]];
+    }
+    c->ds.alpha0 = s.vorb[0];
+    if (nq > 0) {
+        HIPCHK(hipMemcpyAsync(const_cast<int4 *>(c->ds.qm), qm.data(), sizeof(int4) * nq,
+                              hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(const_cast<double *>(c->ds.wq), wq.data(), sizeof(double) * nq,
+                              hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+void fill_devsys(dmc_ctx *c) {
+    const System &s = c->sys;
+    DevSys &d = c->ds;
+    d.npart = s.npart;
+    d.n = s.n;
+    d.nq = s.nk - 1;
+    d.ntab = s.ntab;
+    d.nobeta = s.nobeta;
+    d.cutdrift = s.cutdrift;
+    d.fn = s.p.fn;
+    d.nwalk = s.p.nwalk;
+    d.el = s.el;
+    d.eli = s.eli;
+    d.range = s.range;
+    d.scale = s.scale;
+    d.rangeb = s.rangeb;
+    d.scaleb = s.scaleb;
+    d.sigma = s.sigma;
+    d.sig2 = s.sig2;
+    d.sqrtm = s.sqrtm;
+    d.invsqrtm = 1.0 / s.sqrtm;
+    d.invm = 1.0 / s.mass;
+    d.invmu = 1.0 / (s.mass * s.mass / (s.mass + s.mass));
+    d.hbar = s.hbar;
+    d.dt = s.p.dt;
+    d.etrial = s.etrial;
+    d.wtcut = s.p.wtcut;
+    d.dcut = s.dcut_abs;
+    d.nfac = s.nfac;
+    d.twopi_l = 2.0 * 3.14159265358979323846 / s.el;
+    d.popup = s.p.popup;
+    d.popdn = s.p.popdn;
+}
+
+int set_count(dmc_ctx *c, int n) {
+    c->h_nt[0] = n;
+    HIPCHK(hipMemcpyAsync(c->T.nt, c->h_nt, sizeof(int), hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+int read_total(dmc_ctx *c, int &total) {
+    HIPCHK(hipMemcpyAsync(c->h_nt + 1, c->T.nt + 1, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    total = c->h_nt[1];
+    return 0;
+}
+
+// one branch pass: multiplicities already in T.count for the n walkers of src; compacts
+// into dst and returns the new count
+int compact(dmc_ctx *c, const DevState &src, const DevState &dst, int n, int &total) {
+    LAUNCHCHK(launch_scan(c->T, c->st));
+    int st = read_total(c, total);
+    if (st) return st;
+    if (total > c->capacity)
+        return fail(DMC_ERR_OVERFLOW, "stack overflow: population " + std::to_string(total) +
+                                          " exceeds capacity " + std::to_string(c->capacity));
+    LAUNCHCHK(launch_map_gather(c->ds, src, dst, c->T, n, total, c->capacity, c->st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dmc_last_error(void) { return g_err.c_str(); }
+
+int dmc_create(const dmc_params *p, dmc_ctx **out) {
+    if (!p || !out) return fail(DMC_ERR_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(DMC_ERR_NODEVICE, "no HIP device");
+    dmc_ctx *c = new dmc_ctx();
+    std::string err;
+    int st = build_system(*p, c->sys, err);
+    if (st) {
+        delete c;
+        return fail(st, err);
+    }
+    const System &s = c->sys;
+    if (!nup_supported(s.n)) {
+        delete c;
+        return fail(DMC_ERR_ARG, "npart/2 = " + std::to_string(s.n) +
+                                     " has no compiled kernel (supported: 1, 7, 19, 33)");
+    }
+    for (int q = 1; q < s.nk; ++q)
+        for (int d = 0; d < 3; ++d)
+            if (std::abs(s.ik[q][d]) > 4) {
+                delete c;
+                return fail(DMC_ERR_ARG, "k vector component beyond 4 (too many shells)");
+            }
+    if (p->precision != DMC_FP64 && p->precision != DMC_MIXED) {
+        delete c;
+        return fail(DMC_ERR_ARG, "precision must be DMC_FP64 or DMC_MIXED");
+    }
+    if (p->movetype == DMC_ALLPARTICLE && p->precision != DMC_FP64) {
+        delete c;
+        return fail(DMC_ERR_ARG, "all-particle moves run in FP64 only");
+    }
+    c->capacity = p->capacity > 0 ? p->capacity : 4 * (p->nwalk > 0 ? p->nwalk : 1);
+    if (hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return fail(DMC_ERR_HIP, "hipStreamCreate failed");
+    }
+    fill_devsys(c);
+    const int n1 = s.ntab + 1, nq = s.nk - 1;
+    double4 *jt, *bt;
+    float4 *jt32, *bt32;
+    int4 *qm;
+    double *wq;
+    uint64_t *ja1, *jc1, *ja2, *jc2;
+    if (c->alloc(&jt, n1) || c->alloc(&bt, n1) || c->alloc(&jt32, n1) || c->alloc(&bt32, n1) ||
+        c->alloc(&qm, nq) || c->alloc(&wq, nq) || c->alloc(&ja1, kMaxDraw) ||
+        c->alloc(&jc1, kMaxDraw) || c->alloc(&ja2, kMaxDraw) || c->alloc(&jc2, kMaxDraw) ||
+        c->alloc_state(c->A) || c->alloc_state(c->B) || c->alloc(&c->T.count, c->capacity) ||
+        c->alloc(&c->T.offset, c->capacity) || c->alloc(&c->T.map, c->capacity) ||
+        c->alloc(&c->T.copy_irn, c->capacity) || c->alloc(&c->T.est, size_t(c->capacity) * 42) ||
+        c->alloc(&c->T.nt, 4) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
+        c->alloc(&c->est_acc, 42)) {
+        std::string e = g_err;
+        dmc_destroy(c);
+        return fail(DMC_ERR_HIP, e);
+    }
+    c->ds.jt = jt;
+    c->ds.bt = bt;
+    c->ds.jt32 = jt32;
+    c->ds.bt32 = bt32;
+    c->ds.qm = qm;
+    c->ds.wq = wq;
+    c->ds.ja1 = ja1;
+    c->ds.jc1 = jc1;
+    c->ds.ja2 = ja2;
+    c->ds.jc2 = jc2;
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->h_nt), 4 * sizeof(int)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        dmc_destroy(c);
+        return fail(DMC_ERR_HIP, "host/event allocation failed");
+    }
+    std::vector<uint64_t> a1, c1, a2, c2;
+    ran1_jump_table(kMaxDraw, a1, c1);
+    ran2_jump_table(kMaxDraw, a2, c2);
+    int e = 0;
+    if (hipMemcpy(ja1, a1.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(jc1, c1.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ja2, a2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(jc2, c2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->est_acc, 0, 42 * sizeof(double)) != hipSuccess ||
+        hipMemset(c->T.hist, 0, 500 * sizeof(int64_t)) != hipSuccess)
+        e = DMC_ERR_HIP;
+    if (!e) e = upload_tables(c);
+    if (!e) e = upload_q(c);
+    if (e) {
+        std::string m = g_err.empty() ? "upload failed" : g_err;
+        dmc_destroy(c);
+        return fail(DMC_ERR_HIP, m);
+    }
+    *out = c;
+    return DMC_OK;
+}
+
+int dmc_destroy(dmc_ctx *c) {
+    if (!c) return DMC_OK;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) hipFree(p);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->h_nt) hipHostFree(c->h_nt);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return DMC_OK;
+}
+
+int dmc_info(const dmc_ctx *c, double out[8]) {
+    if (!c || !out) return fail(DMC_ERR_ARG, "null argument");
+    const System &s = c->sys;
+    out[0] = s.el;
+    out[1] = s.eunit;
+    out[2] = s.etrial;
+    out[3] = s.nfac;
+    out[4] = s.nk;
+    out[5] = s.ntab0;
+    out[6] = s.sigma;
+    out[7] = c->capacity;
+    return DMC_OK;
+}
+
+int dmc_set_tables(dmc_ctx *c, const double *vt, const double *ut, const double *dut,
+                   const double *d2ut, const double *bt, const double *dbt, const double *d2bt) {
+    if (!c || !vt || !ut || !dut || !d2ut || !bt || !dbt || !d2bt)
+        return fail(DMC_ERR_ARG, "null argument");
+    System &s = c->sys;
+    const int n1 = s.ntab + 1;
+    s.vt.assign(vt, vt + n1);
+    s.ut.assign(ut, ut + n1);
+    s.dut.assign(dut, dut + n1);
+    s.d2ut.assign(d2ut, d2ut + n1);
+    s.bt.assign(bt, bt + n1);
+    s.dbt.assign(dbt, dbt + n1);
+    s.d2bt.assign(d2bt, d2bt + n1);
+    return upload_tables(c);
+}
+
+int dmc_set_params(dmc_ctx *c, const double *vorb, double bbeta, double dbeta) {
+    if (!c || !vorb) return fail(DMC_ERR_ARG, "null argument");
+    System &s = c->sys;
+    s.vorb.assign(vorb, vorb + s.p.norb);
+    if (!s.nobeta && (bbeta != s.b || dbeta != s.d)) {
+        build_beta(s, bbeta, dbeta);
+        int e = upload_tables(c);
+        if (e) return e;
+    }
+    return upload_q(c);
+}
+
+int dmc_load_walkers(dmc_ctx *c, int32_t nw, const double *x, const int64_t *irn) {
+    if (!c || !x || !irn || nw < 0) return fail(DMC_ERR_ARG, "invalid argument");
+    if (nw > c->capacity) return fail(DMC_ERR_OVERFLOW, "more walkers than capacity");
+    const size_t m = size_t(3) * c->sys.npart;
+    HIPCHK(hipMemcpyAsync(c->A.x, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->A.irn, irn, sizeof(int64_t) * nw, hipMemcpyHostToDevice, c->st));
+    LAUNCHCHK(launch_eval(c->sys.n, c->ds, c->A.x, nw, c->A.psil, c->A.is, c->A.dpsi, c->A.d2psi,
+                          c->A.v, c->st));
+    if (nw > 0) hipLaunchKernelGGL(k_init_weights, dim3((nw + 255) / 256), dim3(256), 0, c->st, c->A, nw);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->nw = nw;
+    return DMC_OK;
+}
+
+int dmc_init_walkers(dmc_ctx *c, int64_t seed, int32_t nwalk) {
+    if (!c || nwalk < 0) return fail(DMC_ERR_ARG, "invalid argument");
+    std::vector<double> x;
+    std::vector<int64_t> irn;
+    initial_walkers(c->sys, seed, nwalk, x, irn);
+    return dmc_load_walkers(c, nwalk, x.data(), irn.data());
+}
+
+int dmc_population(const dmc_ctx *c, int32_t *nw) {
+    if (!c || !nw) return fail(DMC_ERR_ARG, "null argument");
+    *nw = c->nw;
+    return DMC_OK;
+}
+
+int dmc_read_walkers(dmc_ctx *c, int32_t cap, int32_t *nw, double *x, double *dpsi, double *sc,
+                     int64_t *irn) {
+    if (!c || !nw) return fail(DMC_ERR_ARG, "null argument");
+    const int n = c->nw;
+    *nw = n;
+    if (cap < n) return fail(DMC_ERR_ARG, "output capacity smaller than the population");
+    const size_t m = size_t(3) * c->sys.npart;
+    if (x) HIPCHK(hipMemcpyAsync(x, c->A.x, sizeof(double) * m * n, hipMemcpyDeviceToHost, c->st));
+    if (dpsi)
+        HIPCHK(hipMemcpyAsync(dpsi, c->A.dpsi, sizeof(double) * m * n, hipMemcpyDeviceToHost, c->st));
+    if (irn) HIPCHK(hipMemcpyAsync(irn, c->A.irn, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->st));
+    if (sc) {
+        std::vector<double> wgt(n), psil(n), d2(n), v(n), wt0(n);
+        std::vector<int> is(n);
+        HIPCHK(hipMemcpyAsync(wgt.data(), c->A.weight, 8 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(psil.data(), c->A.psil, 8 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(d2.data(), c->A.d2psi, 8 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(v.data(), c->A.v, 8 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(wt0.data(), c->A.wt0, 8 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(is.data(), c->A.is, 4 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int w = 0; w < n; ++w) {
+            double *o = sc + 6 * size_t(w);
+            o[0] = wgt[w];
+            o[1] = psil[w];
+            o[2] = d2[w];
+            o[3] = v[w];
+            o[4] = is[w];
+            o[5] = wt0[w];
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_eval(dmc_ctx *c, int32_t nw, const double *x, double *psil, int32_t *is, double *dpsi,
+             double *d2psi, double *v) {
+    if (!c || !x || nw < 0) return fail(DMC_ERR_ARG, "invalid argument");
+    if (nw == 0) return DMC_OK;
+    const size_t m = size_t(3) * c->sys.npart;
+    const size_t bytes = sizeof(double) * (2 * m * nw + 3 * size_t(nw)) + sizeof(int) * nw + 256;
+    int e = c->ensure_scratch(bytes);
+    if (e) return e;
+    double *dx = static_cast<double *>(c->scratch), *dg = dx + m * nw, *dp = dg + m * nw,
+           *dd = dp + nw, *dv = dd + nw;
+    int *di = reinterpret_cast<int *>(dv + nw);
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
+    LAUNCHCHK(launch_eval(c->sys.n, c->ds, dx, nw, dp, di, dg, dd, dv, c->st));
+    if (psil) HIPCHK(hipMemcpyAsync(psil, dp, 8 * nw, hipMemcpyDeviceToHost, c->st));
+    if (is) HIPCHK(hipMemcpyAsync(is, di, 4 * nw, hipMemcpyDeviceToHost, c->st));
+    if (dpsi) HIPCHK(hipMemcpyAsync(dpsi, dg, sizeof(double) * m * nw, hipMemcpyDeviceToHost, c->st));
+    if (d2psi) HIPCHK(hipMemcpyAsync(d2psi, dd, 8 * nw, hipMemcpyDeviceToHost, c->st));
+    if (v) HIPCHK(hipMemcpyAsync(v, dv, 8 * nw, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_ratio1(dmc_ctx *c, int32_t nw, const double *x, const int32_t *ip, const double *xnew,
+               double *ratio) {
+    if (!c || !x || !ip || !xnew || !ratio || nw < 0) return fail(DMC_ERR_ARG, "invalid argument");
+    if (nw == 0) return DMC_OK;
+    for (int w = 0; w < nw; ++w)
+        if (ip[w] < 0 || ip[w] >= c->sys.npart) return fail(DMC_ERR_ARG, "particle index out of range");
+    const size_t m = size_t(3) * c->sys.npart;
+    const size_t bytes = sizeof(double) * (m * nw + 4 * size_t(nw)) + sizeof(int) * nw + 256;
+    int e = c->ensure_scratch(bytes);
+    if (e) return e;
+    double *dx = static_cast<double *>(c->scratch), *dn = dx + m * nw, *dr = dn + 3 * nw;
+    int *di = reinterpret_cast<int *>(dr + nw);
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dn, xnew, sizeof(double) * 3 * nw, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(di, ip, sizeof(int) * nw, hipMemcpyHostToDevice, c->st));
+    LAUNCHCHK(launch_ratio(c->sys.n, c->ds, dx, di, dn, nw, dr, c->st));
+    HIPCHK(hipMemcpyAsync(ratio, dr, 8 * nw, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    const int n = c->nw;
+    if (n <= 0) return fail(DMC_ERR_ARG, "no walkers loaded");
+    int e = set_count(c, n);
+    if (e) return e;
+    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
+    if (c->sys.p.movetype == DMC_SINGLEPARTICLE) {
+        const int k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : c->sys.npart;
+        LAUNCHCHK(launch_sm_step(c->sys.n, c->sys.p.precision == DMC_MIXED, c->ds, c->A, c->B, c->T,
+                                 k, n, c->st));
+    } else {
+        LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
+    }
+    if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->st));
+    LAUNCHCHK(launch_est_reduce(c->T, n, c->est_acc, c->st));
+    int total = 0;
+    e = compact(c, c->B, c->A, n, total);
+    if (e) return e;
+    if (c->timing) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->time_ms += ms;
+        c->launches += 1;
+    }
+    int final_n = total;
+    const double nwalk = c->sys.p.nwalk;
+    if (total > c->sys.p.popup * nwalk || total < c->sys.p.popdn * nwalk) {
+        if (total == 0) return fail(DMC_ERR_ARG, "population died out");
+        e = set_count(c, total);
+        if (e) return e;
+        LAUNCHCHK(launch_branch_count(c->ds, c->A, c->T, 1.0 * c->sys.p.nwalk / total, total, c->st));
+        e = compact(c, c->A, c->B, total, final_n);
+        if (e) return e;
+        std::swap(c->A, c->B);
+    }
+    c->nw = final_n;
+    if (nt) {
+        nt[0] = total;
+        nt[1] = final_n;
+    }
+    return DMC_OK;
+}
+
+int dmc_run(dmc_ctx *c, int32_t nsteps, int32_t *nt) {
+    int32_t tmp[2] = {0, 0};
+    for (int s = 0; s < nsteps; ++s) {
+        int e = dmc_step(c, s == nsteps - 1, tmp);
+        if (e) return e;
+    }
+    if (nt) *nt = c ? c->nw : 0;
+    return DMC_OK;
+}
+
+int dmc_estimators(dmc_ctx *c, double *est, int32_t reset) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (est) HIPCHK(hipMemcpyAsync(est, c->est_acc, 42 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (reset) HIPCHK(hipMemsetAsync(c->est_acc, 0, 42 * sizeof(double), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_wtisto(dmc_ctx *c, int64_t *hist) {
+    if (!c || !hist) return fail(DMC_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpyAsync(hist, c->T.hist, 500 * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (accepted) *accepted = c->acc_moves;
+    if (proposed) *proposed = c->prop_moves;
+    c->acc_moves = c->prop_moves = 0;
+    return DMC_OK;
+}
+
+int dmc_kernel_timing(dmc_ctx *c, int32_t enable) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    c->timing = enable != 0;
+    return DMC_OK;
+}
+
+int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (ms) *ms = c->time_ms;
+    if (launches) *launches = c->launches;
+    if (reset) {
+        c->time_ms = 0.0;
+        c->launches = 0;
+    }
+    return DMC_OK;
+}
+
+int dmc_sync(dmc_ctx *c) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_adjoint(dmc_ctx *c, int32_t, const double *, double *, double *) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    return fail(DMC_ERR_ARG, "adjoint not available");
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Device-side data structures shared by the kernels and the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dmc {
+
+// One system as seen by the kernels (passed by value).
+struct DevSys {
+    int npart, n, nq, ntab;
+    int nobeta, cutdrift, fn, nwalk;
+    double el, eli, range, scale, rangeb, scaleb;
+    double alpha0, sigma, sig2, sqrtm, invsqrtm, invm, invmu, hbar, dt, etrial, wtcut, dcut,
+        nfac, twopi_l, popup, popdn;
+    const double4 *jt;    // Jastrow/potential records {v, u, du, d2u}, [ntab+1]
+    const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
+    const float4 *jt32;   // FP32 copies for the mixed-precision moves
+    const float4 *bt32;
+    const int4 *qm;       // integer q vectors (mx, my, mz, shell), [nq]
+    const double *wq;     // 2 alpha_shell(q), [nq]
+    const uint64_t *ja1;  // ran1 jump-ahead: state after t+1 draws = ja1[t]*s + jc1[t]
+    const uint64_t *jc1;
+    const uint64_t *ja2;  // ran2 jump-ahead
+    const uint64_t *jc2;
+};
+
+// Structure-of-arrays walker population.  Positions and gradients are walker-major,
+// [w][npart][3].
+struct DevState {
+    double *x, *dpsi;
+    double *psil, *d2psi, *v, *weight, *wt0;
+    int *is;
+    uint64_t *irn;
+};
+
+// Per-step scratch.
+struct DevStep {
+    int *count;           // per-walker multiplicity from branch
+    int *offset;          // exclusive scan of count
+    int *map;             // destination -> source walker
+    uint64_t *copy_irn;   // destination RNG state
+    double *est;          // per-walker estimator contributions [w][14][3]
+    int *nt;              // device population counters: nt[0] current, nt[1] new
+    int64_t *hist;        // branch multiplicity histogram [500]
+    int *nacc;            // accepted single-particle moves per walker
+};
+
+}  // namespace dmc

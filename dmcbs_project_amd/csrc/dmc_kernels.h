@@ -1,0 +1,38 @@
+// Kernel entry points and shared device constants.
+#pragma once
+#include "dmc_device.h"
+
+namespace dmc {
+
+constexpr uint64_t kMask48 = (1ULL << 48) - 1ULL;
+constexpr uint64_t kM1 = 44485709377909ULL, kA1 = 96309754297ULL;  // ran1 (ran.f90:15,18)
+constexpr uint64_t kM2 = 34522712143931ULL, kA2 = 55789347517ULL;  // ran2 (ran.f90:36,39)
+constexpr double kPi = 3.14159265358979323846;
+
+// Host launchers (dmc_launch.hip).  Each returns a hipError_t value.
+int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
+                double *dpsi, double *d2psi, double *v, hipStream_t st);
+int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const double *xnew,
+                 int nw, double *ratio, hipStream_t st);
+int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                const DevStep &T, int opcomp, int grid, hipStream_t st);
+int launch_sm_step(int nup, int mixed, const DevSys &S, const DevState &A, const DevState &B,
+                   const DevStep &T, int krefresh, int grid, hipStream_t st);
+int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, const double *dbeta,
+                   double *dlogpsi, double *delocal, hipStream_t st);
+// branching: multiplicities of state A with factor (population control), prefix sums,
+// the destination map and the gather into state B
+int launch_branch_count(const DevSys &S, const DevState &A, const DevStep &T, double factor,
+                        int grid, hipStream_t st);
+int launch_scan(const DevStep &T, hipStream_t st);
+int launch_map_gather(const DevSys &S, const DevState &src, const DevState &dst,
+                      const DevStep &T, int nsrc, int ndst, int capacity, hipStream_t st);
+int launch_est_reduce(const DevStep &T, int grid, double *acc, hipStream_t st);
+
+bool nup_supported(int nup);
+
+}  // namespace dmc
+
+namespace dmc {
+size_t allpart_lds_bytes(int nup, int nq);
+}

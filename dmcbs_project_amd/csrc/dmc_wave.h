@@ -1,0 +1,557 @@
+// Device building blocks of one walker per 64-lane workgroup: LDS layout, the 8 x 8 lane
+// tile, pairing-matrix build and Gauss-Jordan inversion, gradient/laplacian contraction,
+// Jastrow and potential lookups.  Included by the kernel translation units.
+#pragma once
+
+
+#include "dmc_kernels.h"
+
+namespace dmc {
+
+// ------------------------------------------------------------------------------------
+// small helpers
+
+__device__ __forceinline__ double wrapd(double x, const DevSys &S) {
+    return x - S.el * round(x * S.eli);  // Fortran nint: half away from zero
+}
+
+// 4-point Lagrange weights of coshpot.f90:172-179 / betafun.f90:468-475
+__device__ __forceinline__ int lagrange(double s, int ntab, double &w0, double &w1, double &w2,
+                                        double &w3) {
+    int idx = int(s);
+    idx = idx > ntab - 2 ? ntab - 2 : idx;
+    idx = idx < 1 ? 1 : idx;
+    const double t = s - idx;
+    w0 = -t * (t - 1.0) * (t - 2.0) * (1.0 / 6.0);
+    w1 = (t + 1.0) * (t - 1.0) * (t - 2.0) * 0.5;
+    w2 = -(t + 1.0) * t * (t - 2.0) * 0.5;
+    w3 = (t + 1.0) * t * (t - 1.0) * (1.0 / 6.0);
+    return idx;
+}
+
+__device__ __forceinline__ double4 lookup4(const double4 *tab, int idx, double w0, double w1,
+                                           double w2, double w3) {
+    const double4 a = tab[idx - 1], b = tab[idx], c = tab[idx + 1], d = tab[idx + 2];
+    return make_double4(w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x,
+                        w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y,
+                        w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z,
+                        w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w);
+}
+
+// beta(r) and (beta', lap beta) from the beta table (radialfun, betafun.f90:458-479)
+__device__ __forceinline__ double beta_val(const DevSys &S, double r) {
+    if (S.nobeta || r > S.rangeb) return 0.0;
+    double w0, w1, w2, w3;
+    const int idx = lagrange(S.scaleb * r, S.ntab, w0, w1, w2, w3);
+    return w0 * S.bt[idx - 1].x + w1 * S.bt[idx].x + w2 * S.bt[idx + 1].x + w3 * S.bt[idx + 2].x;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// exp(i m theta) from the precomputed powers, |m| <= 4
+__device__ __forceinline__ void cpow(const double (&pc)[5], const double (&ps)[5], int m,
+                                     double &c, double &s) {
+    // selects kept apart by scheduling barriers so that they are not folded into an
+    // indexed load from a private (scratch) array
+    const int am = m < 0 ? -m : m;
+    c = pc[0];
+    s = ps[0];
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+        const bool t = am == k;
+        c = t ? pc[k] : c;
+        s = t ? ps[k] : s;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    s = m < 0 ? -s : s;
+}
+
+// exp(i q.r) for all q of one position
+__device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const double *pos,
+                                          double2 *out, int stride) {
+    double pc[3][5], ps[3][5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double s1, c1;
+        sincos(pos[d] * S.twopi_l, &s1, &c1);
+        pc[d][0] = 1.0; ps[d][0] = 0.0;
+        pc[d][1] = c1; ps[d][1] = s1;
+#pragma unroll
+        for (int m = 2; m < 5; ++m) {
+            pc[d][m] = pc[d][m - 1] * c1 - ps[d][m - 1] * s1;
+            ps[d][m] = ps[d][m - 1] * c1 + pc[d][m - 1] * s1;
+        }
+    }
+    for (int q = 0; q < S.nq; ++q) {
+        const int4 m = qm[q];
+        double cx, sx, cy, sy, cz, sz;
+        cpow(pc[0], ps[0], m.x, cx, sx);
+        cpow(pc[1], ps[1], m.y, cy, sy);
+        cpow(pc[2], ps[2], m.z, cz, sz);
+        const double cxy = cx * cy - sx * sy, sxy = sx * cy + cx * sy;
+        out[q * stride] = make_double2(cxy * cz - sxy * sz, sxy * cz + cxy * sz);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// LDS layout of one walker (offsets in doubles)
+
+template <int NUP>
+struct Layout {
+    static constexpr int TR = (NUP + 7) / 8, R8 = 8 * TR, SP = R8 + 1, NP = 2 * NUP;
+    int xs, dp, xn, qc, prow, mcol, ints, work, sinv, scal, total;
+    __host__ __device__ explicit Layout(int nq) {
+        xs = 0;
+        dp = xs + 3 * NP;
+        xn = dp + 3 * NP;           // one extra position (4) and its phases (2 nq)
+        qc = xn + 4 + 2 * nq;       // per q: wq, gq.x, gq.y, gq.z, lq
+        prow = qc + 5 * nq;
+        mcol = prow + R8;
+        ints = mcol + R8;           // perm[64], pinv[64], qm[nq] (int4)
+        work = ints + 64 + 2 * nq;
+        const int phs = 2 * NP * nq;
+        sinv = work + phs;
+        const int w1 = phs + R8 * SP, w2 = 96 * NUP;
+        scal = work + (w1 > w2 ? w1 : w2);
+        total = scal + 16;
+    }
+};
+
+template <int NUP>
+size_t lds_bytes(int nq) { return sizeof(double) * Layout<NUP>(nq).total; }
+
+// Load q coefficients into LDS (all lanes)
+template <int NUP>
+__device__ __forceinline__ void load_q(const DevSys &S, double *sm, const Layout<NUP> &L,
+                                       int tid) {
+    int4 *qm = reinterpret_cast<int4 *>(sm + L.ints + 64);
+    const double tp = S.twopi_l;
+    for (int q = tid; q < S.nq; q += 64) {
+        const int4 m = S.qm[q];
+        const double w = S.wq[q];
+        const double qx = tp * m.x, qy = tp * m.y, qz = tp * m.z;
+        double *c = sm + L.qc + 5 * q;
+        c[0] = w;
+        c[1] = -w * qx;
+        c[2] = -w * qy;
+        c[3] = -w * qz;
+        c[4] = -w * (qx * qx + qy * qy + qz * qz);
+        qm[q] = m;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Lane tile of the 8 x 8 grid
+template <int NUP>
+struct Tile {
+    static constexpr int TR = (NUP + 7) / 8;
+    int r, c, ri[TR], cj[TR];
+    bool rv[TR], cv[TR];
+    __device__ explicit Tile(int tid) {
+        r = tid >> 3;
+        c = tid & 7;
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            const int i = r + 8 * a;
+            rv[a] = i < NUP;
+            ri[a] = rv[a] ? i : NUP - 1;
+            const int j = c + 8 * a;
+            cv[a] = j < NUP;
+            cj[a] = cv[a] ? j : NUP - 1;
+        }
+    }
+};
+
+struct DetInfo {
+    double logdet;
+    int sgn;
+    bool singular;
+};
+
+// Phases A-D: plane-wave phases of the positions in LDS (xs), the pairing matrix, its
+// inverse, log|det| and sign.  Leaves the phases (work area) and the inverse (sinv,
+// Sinv(j, i) at [j * SP + i]) in LDS.
+template <int NUP>
+__device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> &L,
+                                 const Tile<NUP> &T, int tid) {
+    constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
+    const int nq = S.nq;
+    const int r = T.r, c = T.c;
+    const double *xs = sm + L.xs;
+    double2 *ph = reinterpret_cast<double2 *>(sm + L.work);
+    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
+    int *perm = reinterpret_cast<int *>(sm + L.ints);
+    int *pinv = perm + 64;
+    const double *qc = sm + L.qc;
+    const int *ri = T.ri, *cj = T.cj;
+    const bool *rv = T.rv, *cv = T.cv;
+
+    // Phase A: plane-wave phases of every particle
+    for (int p = tid; p < NP; p += 64) phases_of(S, qm, xs + 3 * p, ph + p * nq, 1);
+    __syncthreads();
+
+    // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij)
+    double Sm[TR][TR];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b) {
+            const double *xi = xs + 3 * ri[a], *xj = xs + 3 * (NUP + cj[b]);
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double dx = wrapd(xi[d] - xj[d], S);
+                r2 += dx * dx;
+            }
+            Sm[a][b] = S.alpha0 + beta_val(S, sqrt(r2));
+        }
+    for (int q = 0; q < nq; ++q) {
+        const double w = qc[5 * q];
+        double2 zi[TR], zj[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            zi[a] = ph[ri[a] * nq + q];
+            zi[a].x *= w;
+            zi[a].y *= w;
+            zj[a] = ph[(NUP + cj[a]) * nq + q];
+        }
+#pragma unroll
+        for (int a = 0; a < TR; ++a)
+#pragma unroll
+            for (int b = 0; b < TR; ++b) Sm[a][b] += zi[a].x * zj[b].x + zi[a].y * zj[b].y;
+    }
+
+    // Phase C: Gauss-Jordan inversion with partial pivoting, rows never moved
+    double *prow = sm + L.prow, *mcol = sm + L.mcol;
+    unsigned used = 0;
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+        if (!rv[a]) used |= 1u << a;
+    double prod = 1.0;
+    int expo = 0, sgn = 1;
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < NUP; ++k) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int bk = k >> 3, ck = k & 7;
+        double best = -1.0;
+        int brow = 1 << 20;
+        if (c == ck) {
+#pragma unroll
+            for (int a = 0; a < TR; ++a) {
+                const double v = fabs(Sm[a][bk]);
+                if (!((used >> a) & 1u) && v > best) {
+                    best = v;
+                    brow = r + 8 * a;
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 8; m <= 32; m <<= 1) {
+            const double ob = __shfl_xor(best, m, 64);
+            const int orow = __shfl_xor(brow, m, 64);
+            if (ob > best || (ob == best && orow < brow)) {
+                best = ob;
+                brow = orow;
+            }
+        }
+        const int p = __builtin_amdgcn_readlane(brow, ck);
+        if (r == (p & 7)) {
+            const int ap = p >> 3;
+#pragma unroll
+            for (int a = 0; a < TR; ++a)
+                if (a == ap) {
+#pragma unroll
+                    for (int b = 0; b < TR; ++b) prow[c + 8 * b] = Sm[a][b];
+                }
+        }
+        if (c == ck) {
+#pragma unroll
+            for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][bk];
+        }
+        if (tid == 0) {
+            perm[k] = p;
+            pinv[p] = k;
+        }
+        __syncthreads();
+        const double piv = mcol[p];
+        singular |= (piv == 0.0);
+        const double rp = 1.0 / piv;
+        sgn = piv < 0.0 ? -sgn : sgn;
+        prod *= fabs(piv);
+        if ((k & 7) == 7) {
+            int e;
+            prod = frexp(prod, &e);
+            expo += e;
+        }
+        double f[TR], pv[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a] * rp;
+#pragma unroll
+        for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            const bool isp = (r + 8 * a) == p;
+#pragma unroll
+            for (int b = 0; b < TR; ++b) {
+                const bool jk = (b == bk) && (c == ck);
+                const double upd = jk ? -f[a] : Sm[a][b] - f[a] * pv[b];
+                const double prw = jk ? rp : pv[b] * rp;
+                Sm[a][b] = isp ? prw : upd;
+            }
+            if (isp) used |= 1u << a;
+        }
+        __syncthreads();
+    }
+
+    // permutation parity: n - #cycles
+    int ismin = 0;
+    if (tid < NUP) {
+        ismin = 1;
+        int s = perm[tid];
+        for (int it = 0; it < NUP && s != tid; ++it) {
+            if (s < tid) { ismin = 0; break; }
+            s = perm[s];
+        }
+    }
+    const int ncyc = __popcll(__ballot(ismin));
+    if ((NUP - ncyc) & 1) sgn = -sgn;
+    const double logdet = singular ? -1.0e30 : log(prod) + expo * 0.69314718055994530942;
+    if (singular) sgn = 1;
+
+    // Phase D: scatter the inverse: Sinv(pinv[alpha], perm[beta]) = B(alpha, beta)
+    double *sinv = sm + L.sinv;
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+            if (rv[a] && cv[b]) sinv[pinv[ri[a]] * SP + perm[cj[b]]] = Sm[a][b];
+    __syncthreads();
+    return DetInfo{logdet, sgn, singular};
+}
+
+// Phases E-G: gradients and laplacian of log det through the inverse, Jastrow and
+// potential, and the hpsi combination (wavefunction.f90:171-201).  Results in LDS:
+// scal[0..4] = psil, d2psi, v, is, log|det|; dp = dpsi.  Consumes the phases and the
+// inverse (their area is reused for the partial sums).
+template <int NUP>
+__device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, const Tile<NUP> &T,
+                         const DetInfo &D, int tid) {
+    constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
+    const int nq = S.nq;
+    const int r = T.r, c = T.c;
+    const double *xs = sm + L.xs;
+    const double2 *ph = reinterpret_cast<const double2 *>(sm + L.work);
+    const double *qc = sm + L.qc;
+    const double *sinv = sm + L.sinv;
+    const int *ri = T.ri, *cj = T.cj;
+    const bool *rv = T.rv, *cv = T.cv;
+    const bool singular = D.singular;
+    const double logdet = D.logdet;
+    const int sgn = D.sgn;
+    double sv[TR][TR];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+            sv[a][b] = (rv[a] && cv[b] && !singular) ? sinv[cj[b] * SP + ri[a]] : 0.0;
+    double gu[TR][3], gd[TR][3], ju[TR][3], jd[TR][3];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gu[a][d] = gd[a][d] = ju[a][d] = jd[a][d] = 0.0;
+    double lap = 0.0, U = 0.0, V = 0.0, D2U = 0.0;
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b) {
+            if (!(rv[a] && cv[b])) continue;
+            const double *xi = xs + 3 * ri[a], *xj = xs + 3 * (NUP + cj[b]);
+            double dx[3], r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                dx[d] = wrapd(xi[d] - xj[d], S);
+                r2 += dx[d] * dx[d];
+            }
+            const double rr = sqrt(r2);
+            if (!S.nobeta && rr <= S.rangeb) {
+                double w0, w1, w2, w3;
+                const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
+                const double4 bv = lookup4(S.bt, idx, w0, w1, w2, w3);
+                const double t = sv[a][b] * bv.y / rr;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    gu[a][d] += t * dx[d];
+                    gd[b][d] -= t * dx[d];
+                }
+                lap += sv[a][b] * bv.z;
+            }
+            const double rj = fmax(1.0e-14, rr);
+            if (rj < S.range) {
+                double w0, w1, w2, w3;
+                const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+                const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
+                V += jv.x;
+                U += jv.y;
+                const double t = jv.z / (rj * S.sqrtm);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    ju[a][d] += dx[d] * t;
+                    jd[b][d] -= dx[d] * t;
+                }
+                D2U += (jv.w + 2.0 * jv.z / rj) * S.invmu;
+            }
+        }
+    for (int q = 0; q < nq; ++q) {
+        const double *cq = qc + 5 * q;
+        double2 zi[TR], zj[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            zi[a] = ph[ri[a] * nq + q];
+            zj[a] = ph[(NUP + cj[a]) * nq + q];
+        }
+        double X[TR], Y[TR], Z[TR], W[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            X[a] = Y[a] = Z[a] = W[a] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < TR; ++a)
+#pragma unroll
+            for (int b = 0; b < TR; ++b) {
+                X[a] += sv[a][b] * zj[b].x;
+                Y[a] += sv[a][b] * zj[b].y;
+                Z[b] += sv[a][b] * zi[a].y;
+                W[b] += sv[a][b] * zi[a].x;
+            }
+        double ls = 0.0;
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            const double t = zi[a].y * X[a] - zi[a].x * Y[a];
+            gu[a][0] += cq[1] * t;
+            gu[a][1] += cq[2] * t;
+            gu[a][2] += cq[3] * t;
+            ls += zi[a].x * X[a] + zi[a].y * Y[a];
+            const double u = zj[a].x * Z[a] - zj[a].y * W[a];
+            gd[a][0] -= cq[1] * u;
+            gd[a][1] -= cq[2] * u;
+            gd[a][2] -= cq[3] * u;
+        }
+        lap += cq[4] * ls;
+    }
+    __syncthreads();  // done with ph and sinv: reuse the work area for partial sums
+
+    // Phase G: reduce per-particle partial sums across the lane grid
+    double *pu = sm + L.work;            // [6][NUP][8]  up particles, over c
+    double *pd = pu + 48 * NUP;          // [6][NUP][8]  down particles, over r
+#pragma unroll
+    for (int a = 0; a < TR; ++a) {
+        if (rv[a]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                pu[(d * NUP + ri[a]) * 8 + c] = gu[a][d];
+                pu[((3 + d) * NUP + ri[a]) * 8 + c] = ju[a][d];
+            }
+        }
+        if (cv[a]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                pd[(d * NUP + cj[a]) * 8 + r] = gd[a][d];
+                pd[((3 + d) * NUP + cj[a]) * 8 + r] = jd[a][d];
+            }
+        }
+    }
+    __syncthreads();
+    double du2 = 0.0, dudp = 0.0;
+    for (int p = tid; p < NP; p += 64) {
+        const double *src = p < NUP ? pu : pd;
+        const int pp = p < NUP ? p : p - NUP;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double g = 0.0, j = 0.0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                g += src[(d * NUP + pp) * 8 + l];
+                j += src[((3 + d) * NUP + pp) * 8 + l];
+            }
+            g = singular ? 0.0 : g * S.invsqrtm;
+            sm[L.dp + 3 * p + d] = g + j;
+            du2 += j * j;
+            dudp += j * g;
+        }
+    }
+    lap = wave_sum(lap);
+    U = wave_sum(U);
+    V = wave_sum(V);
+    D2U = wave_sum(D2U);
+    du2 = wave_sum(du2);
+    dudp = wave_sum(dudp);
+    if (tid == 0) {
+        const double d2phi = singular ? 0.0 : 2.0 * lap * S.invm;
+        sm[L.scal + 0] = logdet + U;
+        sm[L.scal + 1] = d2phi + D2U + du2 + 2.0 * dudp;
+        sm[L.scal + 2] = V;
+        sm[L.scal + 3] = double(sgn);
+        sm[L.scal + 4] = logdet;
+    }
+    __syncthreads();
+}
+
+template <int NUP>
+__device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
+                                           int tid) {
+    const Tile<NUP> T(tid);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
+    grad_lap<NUP>(S, sm, L, T, D, tid);
+}
+
+// Jastrow ratio exp(sum_j u(|x' - r_j|) - u(|r_i - r_j|)) over the opposite-spin partners
+// of particle i (getphione, coshpot.f90:207-264); lanes over partners, wave-reduced.
+template <int NUP>
+__device__ double jastrow_logratio(const DevSys &S, const double *xs, int i, const double *xn,
+                                   int tid) {
+    constexpr int NP = 2 * NUP;
+    double du = 0.0;
+    for (int j = tid; j < NP; j += 64) {
+        if ((j < NUP) == (i < NUP)) continue;
+        for (int pass = 0; pass < 2; ++pass) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double xi = pass ? xn[d] : xs[3 * i + d];
+                const double dx = wrapd(xi - xs[3 * j + d], S);
+                r2 += dx * dx;
+            }
+            const double rj = fmax(1.0e-14, sqrt(r2));
+            if (rj < S.range) {
+                double w0, w1, w2, w3;
+                const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+                const double u = lookup4(S.jt, idx, w0, w1, w2, w3).y;
+                du += pass ? u : -u;
+            }
+        }
+    }
+    return wave_sum(du);
+}
+
+// pair function phi(xa - xb) from the phases of both positions (allpair element)
+__device__ __forceinline__ double phi_pair(const DevSys &S, const double *qc, const double2 *za,
+                                           const double2 *zb, const double *xa,
+                                           const double *xb) {
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double dx = wrapd(xa[d] - xb[d], S);
+        r2 += dx * dx;
+    }
+    double f = S.alpha0 + beta_val(S, sqrt(r2));
+    for (int q = 0; q < S.nq; ++q) f += qc[5 * q] * (za[q].x * zb[q].x + za[q].y * zb[q].y);
+    return f;
+}
+
+}  // namespace dmc

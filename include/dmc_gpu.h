@@ -1,0 +1,131 @@
+/* dmc_gpu.h -- C-ABI of the MI355X walker-propagation engine for DMCBS.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status returns (0 = success).
+ * Host pointers are caller-owned and copied; the walker population lives in device memory
+ * owned by the context.  One host thread per context.  Positions are [npart][3] per walker
+ * (particle-major, x,y,z, up spins first), the memory order of the reference's x(3,npart).
+ *
+ * Each entry point names the reference interface it replaces (file:line in the DMCBS
+ * tree).  The reference exposes Fortran module procedures; its Fortran host binds these
+ * functions through iso_c_binding (see INTEGRATION.md for the interface blocks).
+ */
+#ifndef DMC_GPU_H
+#define DMC_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMC_MAXORB 16
+
+/* status codes (replace the reference's `stop` paths) */
+enum {
+    DMC_OK = 0,
+    DMC_ERR_ARG = 1,          /* invalid argument or unsupported configuration */
+    DMC_ERR_HIP = 2,          /* HIP runtime error */
+    DMC_ERR_OVERFLOW = 3,     /* population exceeds capacity (stack.f90:88-91) */
+    DMC_ERR_SETUP = 4,        /* "Too many states" (kshell.f90:137-140) or RK4 bisection
+                                 failure (rungekutta.f90:336-339) */
+    DMC_ERR_NODEVICE = 5
+};
+
+/* precision of the propagator */
+enum { DMC_FP64 = 0, DMC_MIXED = 1 };
+/* move type: the reference's all-particle move (step.f90:56-76), or single-particle moves
+ * with Sherman-Morrison inverse updates and an FP64 LU refresh every krefresh moves */
+enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1 };
+
+/* Run parameters: the reference's deck (dmcbcs.f90:43-66, psislaterbcs.f90:47-68,
+ * betafun.f90:304-306, coshpot.f90:35-41) plus the device options. */
+typedef struct dmc_params {
+    int32_t npart;      /* total particles, nup = ndn = npart/2 */
+    int32_t norb;       /* s-wave k-shells */
+    int32_t ntab;       /* table points */
+    int32_t fn;         /* fixed node */
+    int32_t units;      /* energies in units of E_F */
+    int32_t nwalk;      /* target population */
+    int32_t capacity;   /* device walker capacity (0: 4 * nwalk) */
+    int32_t precision;  /* DMC_FP64 or DMC_MIXED */
+    int32_t movetype;   /* DMC_ALLPARTICLE or DMC_SINGLEPARTICLE */
+    int32_t krefresh;   /* single-particle moves between FP64 LU refreshes (0: npart) */
+    double rho, hbar, mass, dt, etrial, dcut, wtcut, popup, popdn;
+    double bbeta, dbeta, bfact, v0, amu, cjas1, cjas2, djas, v0op, v0eq;
+    double vorb[DMC_MAXORB];
+} dmc_params;
+
+typedef struct dmc_ctx dmc_ctx;
+
+/* Set-up: replaces setpsi/setupbcs (wavefunction.f90:13-96, psislaterbcs.f90:23-153),
+ * setbeta (betafun.f90:297-357), setphi (coshpot.f90:14-143) and setstep (step.f90:11-41).
+ * Builds the k shells, beta and Jastrow/potential tables on the host and uploads them. */
+int dmc_create(const dmc_params *p, dmc_ctx **out);
+int dmc_destroy(dmc_ctx *c);
+/* el, eunit, etrial (internal units), nfac, nk, ntab0, sigma, capacity */
+int dmc_info(const dmc_ctx *c, double out[8]);
+const char *dmc_last_error(void);
+
+/* Replace the tables built by dmc_create, e.g. with the host's own setphi/setbeta tables:
+ * v, u, du, d2u (coshpot.f90 vtab/utab/dutab/d2utab(1,0:ntab)) and b, db, d2b
+ * (betafun.f90 swbtab/swdbtab/swd2btab(0:ntab)), each ntab+1 values. */
+int dmc_set_tables(dmc_ctx *c, const double *vt, const double *ut, const double *dut,
+                   const double *d2ut, const double *bt, const double *dbt,
+                   const double *d2bt);
+/* setdetparam/setbcspar (wavefunction.f90:231-258, psislaterbcs.f90:401-419):
+ * alpha_1..alpha_norb, and beta's (b, d) (initbeta, betafun.f90:359-410) */
+int dmc_set_params(dmc_ctx *c, const double *vorb, double bbeta, double dbeta);
+
+/* Walker store (stack.f90 create/push, dmcbcs.f90:264-345): load nw walkers and run the
+ * initial hpsi pass on the device; wt0 = sign, weight = 1. */
+int dmc_load_walkers(dmc_ctx *c, int32_t nw, const double *x, const int64_t *irn);
+/* Initial walkers from one uniform configuration (dmcbcs.f90:237-301), generated on the
+ * host exactly as the reference does, then loaded as above. */
+int dmc_init_walkers(dmc_ctx *c, int64_t seed, int32_t nwalk);
+/* Read the resident population (pop/numstack, stack.f90:97-119).  sc[w][6] = weight, psil,
+ * d2psi, v, is, wt0.  Any output pointer may be NULL. */
+int dmc_read_walkers(dmc_ctx *c, int32_t cap, int32_t *nw, double *x, double *dpsi,
+                     double *sc, int64_t *irn);
+int dmc_population(const dmc_ctx *c, int32_t *nw);
+
+/* hpsi on a batch of configurations (wavefunction.f90:161-201): psil, is, dpsi[nw][npart][3],
+ * d2psi, v.  Runs on the device in FP64. */
+int dmc_eval(dmc_ctx *c, int32_t nw, const double *x, double *psil, int32_t *is,
+             double *dpsi, double *d2psi, double *v);
+/* move1 (wavefunction.f90:393-404): psi(x with particle ip[w] at xnew[w])/psi(x) */
+int dmc_ratio1(dmc_ctx *c, int32_t nw, const double *x, const int32_t *ip,
+               const double *xnew, double *ratio);
+
+/* One DMC step of the resident population: step1 (step.f90:43-285, idmc=1), branch
+ * (step.f90:287-315) and the population control of dmcbcs.f90:441-445.  opcomp = 1
+ * accumulates the energy estimators as step1 does on the last step of a block.
+ * nt[0] = population after branching, nt[1] = after population control. */
+int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt);
+/* Run nsteps steps (opcomp on the last), returning the final population. */
+int dmc_run(dmc_ctx *c, int32_t nsteps, int32_t *nt);
+/* Estimator accumulators est[14][3] = (sum val*wt, sum val^2*wt, sum wt) in the order of
+ * dmcbcs.f90:188-204 (addval, estimator.f90:72-82); reset = 1 zeroes them afterwards. */
+int dmc_estimators(dmc_ctx *c, double *est, int32_t reset);
+/* Multiplicity histogram of branch (wtisto, step.f90:310) for iwt = 0..499 */
+int dmc_wtisto(dmc_ctx *c, int64_t *hist);
+/* Accepted single-particle moves and proposed moves since the last call (single-particle
+ * mode only). */
+int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed);
+
+/* Adjoint: d log psi / d theta and d E_L / d theta for theta = (alpha_1..alpha_norb, b, d)
+ * on a batch of configurations (getderpsi/getderbcs psislaterbcs.f90:449-524 with derbeta
+ * betafun.f90:481-612; the reference computes no d E_L / d theta).  The inverse is
+ * recomputed, not stored.  dlogpsi and delocal are [nw][norb+2]; delocal may be NULL. */
+int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double *delocal);
+
+/* Timing of the dominant kernel (the fused step kernel) with HIP events recorded on the
+ * stream it is launched on: enable = 1 starts collecting, 0 stops.  Returns the summed
+ * milliseconds and the number of launches since the last reset. */
+int dmc_kernel_timing(dmc_ctx *c, int32_t enable);
+int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset);
+/* Synchronise the context's stream. */
+int dmc_sync(dmc_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
