@@ -1,0 +1,219 @@
+"""Throughput of the DMC walker-propagation path on one MI355X.
+
+Headline workload: the N=66 unitary Fermi gas (33 up + 33 down, cosh potential with
+mu = 40, 5 k-shells, 1024-point tables), 32768 walkers from seed 7, tau = 0.001, each step a
+sweep of 66 single-particle moves with Sherman-Morrison inverse updates and an FP64 LU
+refresh every 66 moves, then FP64 local energy, weights, branching and population control.
+The mixed-precision propagator (FP32 moves, FP64 refresh/energies/weights/sign test) is the
+reported value; the plain FP64 propagator runs beside it for the ratio, and the per-step
+local-energy error of the mixed path against FP64 is measured on the same configurations.
+
+    python bench.py [--steps K] [--warmup W] [--workload mixed|fp64|allparticle]
+
+Prints one JSON line.  A "step" is one DMC step of the whole resident population; value is
+walker-steps per second (sum of the populations of the timed steps / wall time), with the
+walkers resident in HBM before the timed region starts.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+N, NORB, NWALK, DT, SEED = 66, 5, 32768, 0.001, 7
+DCUT_SM = 2.0
+
+
+def flops_per_walker_step(mode, n=N // 2, k=N, kq=17):
+    """Algorithmic flop counts per walker-step (C = 2K - 1 plane-wave columns)."""
+    c = 2 * kq - 1
+    if mode == "allparticle":
+        return 10 * n * n * c + 2 * n ** 3 + 16 * n * n + 70 * n * n + 90 * n * n
+    nn = 2 * n
+    return (nn * (8 * n * c + 4 * n * n + 164 * n) + (nn / k) * (2 * n ** 3 + 2 * n * n * c)
+            + nn * (2 * n * c + 2 * n) + 16 * n * n)
+
+
+def make_engine(mode, nwalk=NWALK, krefresh=N):
+    from dmcbs_project_amd import Engine, DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE
+    prec = DMC_MIXED if mode == "mixed" else DMC_FP64
+    move = DMC_ALLPARTICLE if mode == "allparticle" else DMC_SINGLEPARTICLE
+    # single-particle moves cut the drift at 2 sigma per component (the reference's dcut,
+    # step.f90:32-37); the all-particle replay keeps the reference deck's dcut = -1
+    dcut = -1.0 if mode == "allparticle" else DCUT_SM
+    e = Engine(npart=N, norb=NORB, nwalk=nwalk, dt=DT, capacity=2 * nwalk, precision=prec,
+               movetype=move, krefresh=krefresh, dcut=dcut)
+    e.init_walkers(SEED)
+    return e
+
+
+def timed_run(mode, steps, warmup):
+    e = make_engine(mode)
+    for _ in range(warmup):
+        e.step()
+    e.sync()
+    e.kernel_timing(True)
+    e.kernel_times(reset=True)
+    pops = []  # populations entering each timed step (the walkers its kernel processes)
+    nw = e.numstack()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pops.append(nw)
+        nw = e.step()[1]
+    e.sync()
+    wall = time.perf_counter() - t0
+    kms, launches = e.kernel_times(reset=True)
+    e.kernel_timing(False)
+    ws = float(np.sum(pops))
+    res = dict(mode=mode, walker_steps=ws, wall=wall, value=ws / wall,
+               ms_per_step=1e3 * wall / steps, kernel_ms=kms / max(launches, 1),
+               launches=launches, pop_mean=float(np.mean(pops)))
+    acc = e.acceptance()
+    if acc[1]:
+        res["acceptance"] = acc[0] / acc[1]
+    e.close()
+    return res
+
+
+def elocal_error(nconf=4096):
+    """Per-walker local energy of one mixed-precision step against the FP64 step from the
+    same configurations and random streams (relative error, max and median)."""
+    em = make_engine("mixed", nwalk=nconf)
+    ef = make_engine("fp64", nwalk=nconf)
+    for _ in range(3):  # move away from the initial permutations, identically in both
+        em.step()
+        ef.step()
+    wm, wf = em.walkers(), ef.walkers()
+    # evaluate both populations' final configurations in FP64 and compare the local energy
+    # the mixed kernel stored against the FP64 evaluation of its own configuration
+    m = min(len(wm["irn"]), 4096)
+    xm = wm["x"][:m]
+    ref = ef.hpsi(xm)
+    el_mixed = -0.5 * wm["d2psi"][:m] + wm["v"][:m]
+    el_ref = -0.5 * ref["d2psi"] + ref["v"]
+    rel = np.abs(el_mixed - el_ref) / np.maximum(np.abs(el_ref), 1e-300)
+    em.close()
+    ef.close()
+    return dict(max=float(rel.max()), median=float(np.median(rel)), n=int(m))
+
+
+def cpu_reference_baseline(seconds=20.0):
+    """The reference's own FP64 CPU path (oracle/_ref/ref_driver, all-particle DMC, nompi)
+    on the host cores: one single-threaded process per core on a bounded sample."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    from oracle import refdeck
+    cores = min(16, len(os.sched_getaffinity(0)))
+    nwalk, nstep = 64, 4
+    deck, _ = refdeck.make_deck(npart=N, norb=NORB, nwalk=nwalk, nav=1, nstep=nstep, dt=DT,
+                                seed=SEED)
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        np.array([0], "<i8").tofile(fin)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([drv, "dmc", fin, os.path.join(td, f"o{i}.bin")],
+                                  stdin=subprocess.PIPE, stdout=subprocess.DEVNULL, cwd=td,
+                                  env=env) for i in range(cores)]
+        for p in procs:
+            p.stdin.write(deck.encode())
+            p.stdin.close()
+        ws = 0
+        for i, p in enumerate(procs):
+            p.wait(timeout=600)
+            raw = np.fromfile(os.path.join(td, f"o{i}.bin"), "<i8")
+            pops = raw[1:1 + 2 * nstep].reshape(nstep, 2)[:, 1]
+            ws += int(pops.sum())
+        wall = time.perf_counter() - t0
+    return dict(value=ws / wall, unit="walker-steps/s", cores=cores, kind="reference",
+                sample=f"reference FP64 all-particle DMC (nompi, amdflang -O2, OpenBLAS 1 "
+                       f"thread): {cores} processes x {nwalk} walkers x {nstep} steps at N=66")
+
+
+def cpu_port_baseline():
+    """The oracle's FP64 single-particle-move sweep (the workload's algorithm) on all host
+    cores (OpenMP), bounded sample."""
+    import oracle
+    cores = min(16, len(os.sched_getaffinity(0)))
+    s = oracle.System(npart=N, norb=NORB, nwalk=NWALK, dt=DT, dcut=DCUT_SM)
+    nw = 8 * cores
+    x, irn = s.init_walkers(SEED, nw)
+    dpsi, sc = s.init_hpsi(x)
+    t0 = time.perf_counter()
+    s.sm_step1(x, dpsi, sc, irn, N, nthreads=cores)
+    wall = time.perf_counter() - t0
+    return dict(value=nw / wall, unit="walker-steps/s", cores=cores, kind="port",
+                sample=f"oracle FP64 single-particle sweep: {nw} walkers x 1 step at N=66")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "fp64", "allparticle"])
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (walkers shard "
+              "trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    print(f"bench: workload={a.workload} N={N} walkers={NWALK} steps={a.steps} "
+          f"warmup={a.warmup}", file=sys.stderr, flush=True)
+    main_run = timed_run(a.workload, a.steps, a.warmup)
+    print(f"bench: {a.workload}: {main_run}", file=sys.stderr, flush=True)
+    extra = {}
+    if a.workload == "mixed":
+        fp = timed_run("fp64", a.steps, a.warmup)
+        print(f"bench: fp64: {fp}", file=sys.stderr, flush=True)
+        extra["fp64_gpu_value"] = fp["value"]
+        extra["mixed_over_fp64"] = main_run["value"] / fp["value"]
+        extra["elocal_rel_err"] = elocal_error()
+    f = flops_per_walker_step(a.workload)
+    achieved = f * main_run["pop_mean"] / (main_run["kernel_ms"] * 1e-3) / 1e12
+    roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                kernel="k_sm_step" if a.workload != "allparticle" else "k_step",
+                kernel_ms=main_run["kernel_ms"], flops_per_walker_step=f,
+                note="FP64 vector-ALU peak (no MFMA is used; 78.6 TF/s is MI355X's FP64 "
+                     "peak for both)")
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_reference_baseline()
+        try:
+            extra["cpu_port"] = cpu_port_baseline()
+        except Exception as ex:  # the oracle is a reported baseline only
+            extra["cpu_port"] = str(ex)
+    wl = {"mixed": "N66 unitary gas, single-particle moves + SM, mixed FP32/FP64, k=66",
+          "fp64": "N66 unitary gas, single-particle moves + SM, FP64, k=66",
+          "allparticle": "N66 unitary gas, all-particle moves (reference replay), FP64"}
+    line = {
+        "metric": "DMC walker-steps/sec at N=66, 32k walkers: mixed vs FP64 GPU; E_L rel err",
+        "value": main_run["value"], "unit": "walker-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": main_run["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if a.workload == "mixed" else "f64",
+        "data": "synthetic: reference initial walkers (one uniform configuration permuted, "
+                "seed 7)",
+        "config": {"workload": wl[a.workload], "npart": N, "norb": NORB, "walkers": NWALK,
+                   "dt": DT, "krefresh": N, "ntab": 1024, "amu": 40.0,
+                   "dcut": -1.0 if a.workload == "allparticle" else DCUT_SM},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    if "acceptance" in main_run:
+        line["acceptance"] = main_run["acceptance"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

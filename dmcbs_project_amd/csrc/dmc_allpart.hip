@@ -78,69 +78,9 @@ __global__ __launch_bounds__(64) void k_step(DevSys S, DevState A, DevState B, D
     }
     __syncthreads();
     hpsi_block<NUP>(S, sm, L, tid);
-    double *xn = B.x + size_t(w) * M, *gn = B.dpsi + size_t(w) * M;
-    for (int t = tid; t < M; t += 64) {
-        xn[t] = sm[L.xs + t];
-        gn[t] = sm[L.dp + t];
-    }
-    double *est = T.est + size_t(w) * 42;
-    if (tid < 42) est[tid] = 0.0;
-    __syncthreads();
-    if (tid == 0) {
-        const double psil2 = sm[L.scal], d2psi2 = sm[L.scal + 1], v2 = sm[L.scal + 2];
-        const int is2 = int(sm[L.scal + 3]);
-        const double en = -S.hbar * d2psi2 + v2;
-        const double ego = -S.hbar * A.d2psi[w] + A.v[w];
-        double wt = exp(-(0.5 * (ego + en) - S.etrial) * S.dt);
-        double weight = fmin(S.wtcut, wt);
-        const bool cross = is2 != A.is[w];
-        if (cross && S.fn) {
-            weight = 0.0;
-            wt = 0.0;
-        }
-        est[12] = cross ? 1.0 : 0.0;  // node crossing (estimator 5): val, val^2, wt
-        est[13] = cross ? 1.0 : 0.0;
-        est[14] = 1.0;
-        const double wt0 = A.wt0[w];
-        const double wtg = wt * double(is2) * wt0;
-        if (opcomp) {
-            // estimators 1-4 and 6-14 of step.f90:142-156 (addval: val*wt, val^2*wt, wt)
-            const double nf = S.nfac, sg = copysign(1.0, wtg);
-            auto put = [&](int e, double val, double wgt) {
-                est[3 * e] = val * wgt;
-                est[3 * e + 1] = val * val * wgt;
-                est[3 * e + 2] = wgt;
-            };
-            put(0, en * nf, wtg);
-            put(1, (en - v2) * nf, wtg);
-            put(2, v2 * nf, wtg);
-            put(3, 0.0, wtg);
-            put(5, 0.5 * (ego + en) * nf, 1.0);
-            put(6, 1.0, 1.0);
-            put(7, is2 * exp(psil2), 1.0);
-            put(8, exp(psil2), 1.0);
-            put(9, double(is2), 1.0);
-            put(10, wt, 1.0);
-            put(11, wtg, 1.0);
-            put(12, sg, 1.0);
-            put(13, sg, wtg);
-        }
-        // new RNG state after gaussian(M), then the branch draw
-        uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;
-        s1 = (s1 * kM1 + kA1) & kMask48;
-        const double ub = double(s1 | 1ull) * 0x1p-48;
-        const int iwt = int(weight + ub);
-        T.count[w] = iwt;
-        if (iwt >= 0 && iwt < 500)
-            atomicAdd(reinterpret_cast<unsigned long long *>(&T.hist[iwt]), 1ull);
-        B.irn[w] = s1;
-        B.psil[w] = psil2;
-        B.d2psi[w] = d2psi2;
-        B.v[w] = v2;
-        B.is[w] = is2;
-        B.wt0[w] = wt0;
-        B.weight[w] = weight;
-    }
+    // new RNG state after gaussian(M)
+    const uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;
+    finish_walker<NUP>(S, sm, L, A, B, T, w, tid, s1, opcomp);
 }
 
 // single-particle ratio psi(x')/psi(x) (move1, wavefunction.f90:393-404): determinant

@@ -89,7 +89,7 @@ struct dmc_ctx {
     }
     int ensure_scratch(size_t bytes) {
         if (bytes <= scratch_bytes) return 0;
-        if (scratch) hipFree(scratch);
+        if (scratch) (void)hipFree(scratch);
         scratch = nullptr;
         scratch_bytes = 0;
         hipError_t e = hipMalloc(&scratch, bytes);
@@ -240,6 +240,14 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         delete c;
         return fail(DMC_ERR_ARG, "precision must be DMC_FP64 or DMC_MIXED");
     }
+    if (p->movetype != DMC_ALLPARTICLE && p->movetype != DMC_SINGLEPARTICLE) {
+        delete c;
+        return fail(DMC_ERR_ARG, "movetype must be DMC_ALLPARTICLE or DMC_SINGLEPARTICLE");
+    }
+    if (p->krefresh < 0) {
+        delete c;
+        return fail(DMC_ERR_ARG, "krefresh must be >= 0");
+    }
     if (p->movetype == DMC_ALLPARTICLE && p->precision != DMC_FP64) {
         delete c;
         return fail(DMC_ERR_ARG, "all-particle moves run in FP64 only");
@@ -263,6 +271,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         c->alloc(&c->T.offset, c->capacity) || c->alloc(&c->T.map, c->capacity) ||
         c->alloc(&c->T.copy_irn, c->capacity) || c->alloc(&c->T.est, size_t(c->capacity) * 42) ||
         c->alloc(&c->T.nt, 4) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
+        c->alloc(&c->T.moves, 2) ||
         c->alloc(&c->est_acc, 42)) {
         std::string e = g_err;
         dmc_destroy(c);
@@ -292,7 +301,8 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         hipMemcpy(ja2, a2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(jc2, c2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->est_acc, 0, 42 * sizeof(double)) != hipSuccess ||
-        hipMemset(c->T.hist, 0, 500 * sizeof(int64_t)) != hipSuccess)
+        hipMemset(c->T.hist, 0, 500 * sizeof(int64_t)) != hipSuccess ||
+        hipMemset(c->T.moves, 0, 2 * sizeof(int64_t)) != hipSuccess)
         e = DMC_ERR_HIP;
     if (!e) e = upload_tables(c);
     if (!e) e = upload_q(c);
@@ -307,13 +317,14 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
 
 int dmc_destroy(dmc_ctx *c) {
     if (!c) return DMC_OK;
-    if (c->st) hipStreamSynchronize(c->st);
-    for (void *p : c->allocs) hipFree(p);
-    if (c->scratch) hipFree(c->scratch);
-    if (c->h_nt) hipHostFree(c->h_nt);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->st) hipStreamDestroy(c->st);
+    // best-effort teardown: errors here have nowhere to go
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->h_nt) (void)hipHostFree(c->h_nt);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return DMC_OK;
 }
@@ -477,7 +488,7 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
     if (c->sys.p.movetype == DMC_SINGLEPARTICLE) {
         const int k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : c->sys.npart;
         LAUNCHCHK(launch_sm_step(c->sys.n, c->sys.p.precision == DMC_MIXED, c->ds, c->A, c->B, c->T,
-                                 k, n, c->st));
+                                 k, opcomp, n, c->st));
     } else {
         LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
     }
@@ -538,9 +549,12 @@ int dmc_wtisto(dmc_ctx *c, int64_t *hist) {
 
 int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed) {
     if (!c) return fail(DMC_ERR_ARG, "null context");
-    if (accepted) *accepted = c->acc_moves;
-    if (proposed) *proposed = c->prop_moves;
-    c->acc_moves = c->prop_moves = 0;
+    int64_t m[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(m, c->T.moves, sizeof(m), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemsetAsync(c->T.moves, 0, sizeof(m), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (accepted) *accepted = m[0];
+    if (proposed) *proposed = m[1];
     return DMC_OK;
 }
 

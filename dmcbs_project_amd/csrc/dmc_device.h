@@ -44,6 +44,7 @@ struct DevStep {
     int *nt;              // device population counters: nt[0] current, nt[1] new
     int64_t *hist;        // branch multiplicity histogram [500]
     int *nacc;            // accepted single-particle moves per walker
+    int64_t *moves;       // accepted and proposed single-particle moves (running totals)
 };
 
 }  // namespace dmc

@@ -17,7 +17,7 @@ int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const
 int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
                 const DevStep &T, int opcomp, int grid, hipStream_t st);
 int launch_sm_step(int nup, int mixed, const DevSys &S, const DevState &A, const DevState &B,
-                   const DevStep &T, int krefresh, int grid, hipStream_t st);
+                   const DevStep &T, int krefresh, int opcomp, int grid, hipStream_t st);
 int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, const double *dbeta,
                    double *dlogpsi, double *delocal, hipStream_t st);
 // branching: multiplicities of state A with factor (population control), prefix sums,

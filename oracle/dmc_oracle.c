@@ -1048,8 +1048,10 @@ static void sm_one(const dmco_ctx *c, const double *x, const double *dpsi, const
         for (int d = 0; d < 3; d++) go[d] = gd[d] + gj[d];
         dmco_gaussian(&s, 3, g3);
         for (int d = 0; d < 3; d++) {
-            /* drift = dpsi * sigma^2 * sqrt(m) with dpsi = grad ln psi / sqrt(m) */
+            /* drift = dpsi * sigma^2 * sqrt(m) with dpsi = grad ln psi / sqrt(m), cut to
+             * +-sigma*dcut when dcut >= 0 (step.f90:66-67) */
             dro[d] = go[d] * sig2;
+            if (c->p.dcut >= 0.0) dro[d] = fmin(sig * c->p.dcut, fmax(-sig * c->p.dcut, dro[d]));
             dx[d] = dro[d] + sig * g3[d];
             xn[d] = wrapd(xi[d] + dx[d], el);
         }
@@ -1063,6 +1065,7 @@ static void sm_one(const dmco_ctx *c, const double *x, const double *dpsi, const
         for (int d = 0; d < 3; d++) {
             gn[d] = gd[d] + gj[d];
             drn[d] = gn[d] * sig2;
+            if (c->p.dcut >= 0.0) drn[d] = fmin(sig * c->p.dcut, fmax(-sig * c->p.dcut, drn[d]));
             logp += (dro[d] + drn[d]) * (dro[d] - drn[d] - 2.0 * dx[d]) / (2.0 * sig2);
         }
         double prob = exp(fmin(0.0, logp));
