@@ -161,7 +161,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "fp64", "allparticle"])
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the named workload (no FP64 leg, no E_L comparison, no CPU)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (walkers shard "
@@ -172,7 +174,9 @@ def main():
     main_run = timed_run(a.workload, a.steps, a.warmup)
     print(f"bench: {a.workload}: {main_run}", file=sys.stderr, flush=True)
     extra = {}
-    if a.workload == "mixed":
+    if a.quick:
+        a.no_cpu = True
+    if a.workload == "mixed" and not a.quick:
         fp = timed_run("fp64", a.steps, a.warmup)
         print(f"bench: fp64: {fp}", file=sys.stderr, flush=True)
         extra["fp64_gpu_value"] = fp["value"]

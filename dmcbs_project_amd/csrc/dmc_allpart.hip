@@ -18,7 +18,7 @@ namespace dmc {
 template <int NUP>
 __global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict__ x, int nw,
                                              double *psil, int *is, double *dpsi,
-                                             double *d2psi, double *v) {
+                                             double *d2psi, double *v, char *sinv) {
     extern __shared__ double sm[];
     const Layout<NUP> L(S.nq);
     const int w = blockIdx.x, tid = threadIdx.x;
@@ -27,7 +27,10 @@ __global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict_
     load_q<NUP>(S, sm, L, tid);
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
     __syncthreads();
-    hpsi_block<NUP>(S, sm, L, tid);
+    const Tile<NUP> T(tid);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
+    if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_bytes(S), tid);
+    grad_lap<NUP>(S, sm, L, T, D, tid);
     for (int t = tid; t < M; t += 64) dpsi[size_t(w) * M + t] = sm[L.dp + t];
     if (tid == 0) {
         psil[w] = sm[L.scal];
@@ -141,10 +144,10 @@ bool nup_supported(int nup) { return nup == 1 || nup == 7 || nup == 19 || nup ==
     }
 
 int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
-                double *dpsi, double *d2psi, double *v, hipStream_t st) {
+                double *dpsi, double *d2psi, double *v, char *sinv, hipStream_t st) {
     if (nw <= 0) return hipSuccess;
     DMC_DISPATCH(nup, hipLaunchKernelGGL(k_eval<N_>, dim3(nw), dim3(64), lds_bytes<N_>(S.nq), st,
-                                         S, x, nw, psil, is, dpsi, d2psi, v));
+                                         S, x, nw, psil, is, dpsi, d2psi, v, sinv));
     return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 // copies are laid out by an exclusive scan of the multiplicities, so the new population is
 // ordered by source walker -- a deterministic order; the reference's LIFO stack order is
 // different, but no walker's evolution depends on order.
-#include "dmc_kernels.h"
+#include "dmc_wave.h"
 
 namespace dmc {
 
@@ -67,7 +67,8 @@ __global__ void k_branch_map(DevState src, DevStep T, int capacity) {
 }
 
 // copy every destination walker from its source; weight becomes sign(1, weight)
-__global__ __launch_bounds__(64) void k_gather(DevState src, DevState dst, DevStep T, int m3) {
+__global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState dst, DevStep T,
+                                               int m3) {
     const int d = blockIdx.x;
     if (d >= T.nt[1]) return;
     const int w = T.map[d];
@@ -76,6 +77,12 @@ __global__ __launch_bounds__(64) void k_gather(DevState src, DevState dst, DevSt
     for (int t = threadIdx.x; t < m3; t += 64) {
         xd[t] = xs[t];
         gd[t] = gs[t];
+    }
+    if (S.keep_sinv) {
+        const size_t nb = sinv_bytes(S) / 4;
+        const uint32_t *a = reinterpret_cast<const uint32_t *>(src.sinv + size_t(w) * nb * 4);
+        uint32_t *b = reinterpret_cast<uint32_t *>(dst.sinv + size_t(d) * nb * 4);
+        for (size_t t = threadIdx.x; t < nb; t += 64) b[t] = a[t];
     }
     if (threadIdx.x == 0) {
         dst.psil[d] = src.psil[w];
@@ -120,7 +127,7 @@ int launch_map_gather(const DevSys &S, const DevState &src, const DevState &dst,
     hipLaunchKernelGGL(k_branch_map, dim3((nsrc + 255) / 256), dim3(256), 0, st, src, T, capacity);
     int e = hipGetLastError();
     if (e) return e;
-    if (ndst > 0) hipLaunchKernelGGL(k_gather, dim3(ndst), dim3(64), 0, st, src, dst, T, 3 * S.npart);
+    if (ndst > 0) hipLaunchKernelGGL(k_gather, dim3(ndst), dim3(64), 0, st, S, src, dst, T, 3 * S.npart);
     return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "dmc_internal.h"
-#include "dmc_kernels.h"
+#include "dmc_wave.h"
 
 namespace dmc {
 
@@ -81,9 +81,11 @@ struct dmc_ctx {
     }
     int alloc_state(DevState &S) {
         const size_t m = size_t(capacity) * 3 * sys.npart;
+        const size_t sb = ds.keep_sinv ? size_t(capacity) * sinv_bytes(ds) : 1;
         if (alloc(&S.x, m) || alloc(&S.dpsi, m) || alloc(&S.psil, capacity) ||
             alloc(&S.d2psi, capacity) || alloc(&S.v, capacity) || alloc(&S.weight, capacity) ||
-            alloc(&S.wt0, capacity) || alloc(&S.is, capacity) || alloc(&S.irn, capacity))
+            alloc(&S.wt0, capacity) || alloc(&S.is, capacity) || alloc(&S.irn, capacity) ||
+            alloc(&S.sinv, sb))
             return DMC_ERR_HIP;
         return 0;
     }
@@ -177,6 +179,8 @@ void fill_devsys(dmc_ctx *c) {
     d.twopi_l = 2.0 * 3.14159265358979323846 / s.el;
     d.popup = s.p.popup;
     d.popdn = s.p.popdn;
+    d.keep_sinv = s.p.movetype == DMC_SINGLEPARTICLE;
+    d.sinv_f32 = s.p.precision == DMC_MIXED;
 }
 
 int set_count(dmc_ctx *c, int n) {
@@ -378,7 +382,7 @@ int dmc_load_walkers(dmc_ctx *c, int32_t nw, const double *x, const int64_t *irn
     HIPCHK(hipMemcpyAsync(c->A.x, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->A.irn, irn, sizeof(int64_t) * nw, hipMemcpyHostToDevice, c->st));
     LAUNCHCHK(launch_eval(c->sys.n, c->ds, c->A.x, nw, c->A.psil, c->A.is, c->A.dpsi, c->A.d2psi,
-                          c->A.v, c->st));
+                          c->A.v, c->ds.keep_sinv ? c->A.sinv : nullptr, c->st));
     if (nw > 0) hipLaunchKernelGGL(k_init_weights, dim3((nw + 255) / 256), dim3(256), 0, c->st, c->A, nw);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
@@ -447,7 +451,7 @@ int dmc_eval(dmc_ctx *c, int32_t nw, const double *x, double *psil, int32_t *is,
            *dd = dp + nw, *dv = dd + nw;
     int *di = reinterpret_cast<int *>(dv + nw);
     HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
-    LAUNCHCHK(launch_eval(c->sys.n, c->ds, dx, nw, dp, di, dg, dd, dv, c->st));
+    LAUNCHCHK(launch_eval(c->sys.n, c->ds, dx, nw, dp, di, dg, dd, dv, nullptr, c->st));
     if (psil) HIPCHK(hipMemcpyAsync(psil, dp, 8 * nw, hipMemcpyDeviceToHost, c->st));
     if (is) HIPCHK(hipMemcpyAsync(is, di, 4 * nw, hipMemcpyDeviceToHost, c->st));
     if (dpsi) HIPCHK(hipMemcpyAsync(dpsi, dg, sizeof(double) * m * nw, hipMemcpyDeviceToHost, c->st));
@@ -486,9 +490,18 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
     if (e) return e;
     if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
     if (c->sys.p.movetype == DMC_SINGLEPARTICLE) {
-        const int k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : c->sys.npart;
-        LAUNCHCHK(launch_sm_step(c->sys.n, c->sys.p.precision == DMC_MIXED, c->ds, c->A, c->B, c->T,
-                                 k, opcomp, n, c->st));
+        // moves in chunks of krefresh, an FP64 refresh of the inverse between chunks, then
+        // the end-of-sweep FP64 evaluation; A.sinv serves as the working inverse
+        const int np = c->sys.npart;
+        const int k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : np;
+        for (int m0 = 0; m0 < np; m0 += k) {
+            const int m1 = m0 + k < np ? m0 + k : np;
+            if (m0 > 0) LAUNCHCHK(launch_sm_refresh(c->sys.n, c->ds, c->B.x, c->A.sinv, c->T, n, c->st));
+            LAUNCHCHK(launch_sm_moves(c->sys.n, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
+                                      m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1,
+                                      c->T, n, c->st));
+        }
+        LAUNCHCHK(launch_sm_end(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
     } else {
         LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
     }

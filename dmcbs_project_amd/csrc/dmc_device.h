@@ -10,6 +10,7 @@ namespace dmc {
 struct DevSys {
     int npart, n, nq, ntab;
     int nobeta, cutdrift, fn, nwalk;
+    int keep_sinv, sinv_f32;  // walkers carry their inverse (single-particle moves), in FP32?
     double el, eli, range, scale, rangeb, scaleb;
     double alpha0, sigma, sig2, sqrtm, invsqrtm, invm, invmu, hbar, dt, etrial, wtcut, dcut,
         nfac, twopi_l, popup, popdn;
@@ -29,6 +30,7 @@ struct DevSys {
 // [w][npart][3].
 struct DevState {
     double *x, *dpsi;
+    char *sinv;  // inverse of the pairing matrix, dense [j][i], FP64 or FP32 (keep_sinv)
     double *psil, *d2psi, *v, *weight, *wt0;
     int *is;
     uint64_t *irn;

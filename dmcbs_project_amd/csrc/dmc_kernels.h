@@ -11,13 +11,22 @@ constexpr double kPi = 3.14159265358979323846;
 
 // Host launchers (dmc_launch.hip).  Each returns a hipError_t value.
 int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
-                double *dpsi, double *d2psi, double *v, hipStream_t st);
+                double *dpsi, double *d2psi, double *v, char *sinv, hipStream_t st);
 int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const double *xnew,
                  int nw, double *ratio, hipStream_t st);
 int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
                 const DevStep &T, int opcomp, int grid, hipStream_t st);
-int launch_sm_step(int nup, int mixed, const DevSys &S, const DevState &A, const DevState &B,
-                   const DevStep &T, int krefresh, int opcomp, int grid, hipStream_t st);
+// single-particle moves m0..m1-1 of every walker, from positions/RNG xin/irn_in and the
+// stored inverse sinv, to xout/irn_out (move kernel)
+int launch_sm_moves(int nup, const DevSys &S, const double *xin, double *xout,
+                    const uint64_t *irn_in, uint64_t *irn_out, char *sinv, int m0, int m1,
+                    const DevStep &T, int grid, hipStream_t st);
+// FP64 rebuild of the inverse of the positions x into sinv (refresh kernel)
+int launch_sm_refresh(int nup, const DevSys &S, const double *x, char *sinv, const DevStep &T,
+                      int grid, hipStream_t st);
+// end of the sweep: FP64 hpsi of B.x, inverse into B.sinv, weights against A, branch draw
+int launch_sm_end(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                  const DevStep &T, int opcomp, int grid, hipStream_t st);
 int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, const double *dbeta,
                    double *dlogpsi, double *delocal, hipStream_t st);
 // branching: multiplicities of state A with factor (population control), prefix sums,

@@ -1,22 +1,28 @@
-// Single-particle-move DMC sweep with Sherman-Morrison inverse updates, one walker per
-// 64-lane workgroup.
+// Single-particle-move DMC sweep with Sherman-Morrison inverse updates.
 //
 // A step moves the particles one at a time in order (0..npart-1).  For particle a the
 // drift comes from grad ln psi at the current configuration (determinant part through
-// column/row a of the inverse, Jastrow part over the partners), the proposal is
+// column/row a of the inverse, Jastrow part over the opposite-spin partners), cut to
+// +-sigma*dcut per component when dcut >= 0 (step.f90:32-37,66-67); the proposal is
 // r' = r + drift + sigma g with g from gaussian(3) of the walker's own ran1 stream, and the
 // move is accepted with the Metropolis probability of the drift-diffusion Green's function
-// (the form of step.f90:80-93), the fixed-node rule rejecting any move that changes the sign
+// (the form of step.f90:80-93); the fixed-node rule rejects any move that changes the sign
 // of the determinant.  An accepted move updates the inverse by a rank-1 Sherman-Morrison
 // step: an up particle changes row a of S(i, j) = phi(r_up_i - r_dn_j), a down particle
-// column a.  Every krefresh moves, and at the end of the sweep, the inverse and log|det|
-// are rebuilt from scratch in FP64; the end-of-sweep rebuild also gives the FP64 local
-// energy, weight, estimators and branch draw exactly as the all-particle step.
+// column a.
 //
-// R = double: every move in FP64.  R = float (mixed precision): the pair functions, Jastrow
-// lookups, gradients and the Sherman-Morrison updates run in FP32 on FP32 copies of the
-// inverse and phases; the refresh, the determinant-ratio sum used for the node test,
-// positions, weights and accumulators stay FP64.
+// Three kernels, so that each gets the registers it needs:
+//  * k_sm_moves: moves m0..m1-1 of one walker per 64-lane workgroup, from the inverse the
+//    walker carries; lanes run over the n opposite-spin partners of the moving particle;
+//  * k_sm_refresh: FP64 rebuild of the inverse (every krefresh moves inside a sweep);
+//  * k_sm_end: at the end of the sweep the FP64 hpsi of the final positions gives the
+//    local energy, weight, estimators and branch draw exactly as the all-particle step, and
+//    the fresh FP64 inverse that the walker carries into the next step.
+//
+// R = double: every move in FP64.  R = float (mixed precision): pair functions, Jastrow
+// lookups, gradients and the Sherman-Morrison updates run in FP32 on an FP32 inverse and
+// FP32 phases; the refresh, the determinant-ratio sum behind the node test, positions,
+// weights and accumulators stay FP64.
 #include "dmc_wave.h"
 
 namespace dmc {
@@ -39,10 +45,17 @@ __device__ __forceinline__ R wsum(R v) {
     return v;
 }
 
-// R-typed table record (v, u, du, d2u) or (b, db, lap b, 0) interpolated at s
-template <typename R>
-__device__ __forceinline__ void tab_at(const double4 *t64, const float4 *t32, R s, int ntab,
-                                       R &x, R &y, R &z, R &w) {
+template <typename R, typename V>
+__device__ __forceinline__ R field(const V &v, int k) {
+    return k == 0 ? R(v.x) : k == 1 ? R(v.y) : k == 2 ? R(v.z) : R(v.w);
+}
+
+// R-typed interpolation of two fields of a table record at s (4-point Lagrange,
+// coshpot.f90:172-192); fields (0,1) = (b, db) of the beta table, (1,2) = (u, du) of the
+// Jastrow table
+template <typename R, int F0, int F1>
+__device__ __forceinline__ void tab2(const double4 *t64, const float4 *t32, R s, int ntab, R &x,
+                                     R &y) {
     int idx = int(s);
     idx = idx > ntab - 2 ? ntab - 2 : idx;
     idx = idx < 1 ? 1 : idx;
@@ -53,106 +66,125 @@ __device__ __forceinline__ void tab_at(const double4 *t64, const float4 *t32, R 
     const R w3 = (t + R(1)) * t * (t - R(1)) * R(1.0 / 6.0);
     if constexpr (sizeof(R) == 8) {
         const double4 a = t64[idx - 1], b = t64[idx], c = t64[idx + 1], d = t64[idx + 2];
-        x = w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x;
-        y = w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y;
-        z = w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z;
-        w = w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w;
+        x = w0 * field<R>(a, F0) + w1 * field<R>(b, F0) + w2 * field<R>(c, F0) + w3 * field<R>(d, F0);
+        y = w0 * field<R>(a, F1) + w1 * field<R>(b, F1) + w2 * field<R>(c, F1) + w3 * field<R>(d, F1);
     } else {
         const float4 a = t32[idx - 1], b = t32[idx], c = t32[idx + 1], d = t32[idx + 2];
-        x = w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x;
-        y = w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y;
-        z = w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z;
-        w = w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w;
+        x = w0 * field<R>(a, F0) + w1 * field<R>(b, F0) + w2 * field<R>(c, F0) + w3 * field<R>(d, F0);
+        y = w0 * field<R>(a, F1) + w1 * field<R>(b, F1) + w2 * field<R>(c, F1) + w3 * field<R>(d, F1);
     }
 }
 
-// LDS of the sweep: the all-particle layout plus, for R = float, FP32 copies of the
-// inverse, the phases and the q coefficients.
+// LDS of the move kernel (offsets in doubles)
 template <int NUP, typename R>
-struct SmLayout {
-    static constexpr int SP = Layout<NUP>::SP, R8 = Layout<NUP>::R8, NP = Layout<NUP>::NP;
-    Layout<NUP> L;
-    int sv, ph, qc, total;  // offsets in doubles
-    __host__ __device__ explicit SmLayout(int nq) : L(nq) {
-        if (sizeof(R) == 8) {
-            sv = L.sinv;
-            ph = L.work;
-            qc = L.qc;
-            total = L.total;
-        } else {
-            sv = L.total;
-            ph = sv + (R8 * SP + 1) / 2;
-            qc = ph + NP * nq;
-            total = qc + (5 * nq + 1) / 2;
-        }
+struct MvLayout {
+    static constexpr int SP = NUP + 1, NP = 2 * NUP;
+    static constexpr int RD = sizeof(R) == 8 ? 1 : 2;  // R values per double
+    int xs, qc, qm, sv, ph, phn, buf, total;
+    __host__ __device__ explicit MvLayout(int nq) {
+        xs = 0;
+        qc = xs + 3 * NP;
+        qm = qc + (5 * nq + RD - 1) / RD;
+        sv = qm + 2 * nq;
+        ph = sv + (NUP * SP + RD - 1) / RD;
+        phn = ph + (2 * NP * nq + RD - 1) / RD;
+        buf = phn + (2 * nq + RD - 1) / RD;
+        total = buf + (3 * SP + RD - 1) / RD;
     }
 };
 
 template <int NUP, typename R>
-size_t sm_lds_bytes(int nq) { return sizeof(double) * SmLayout<NUP, R>(nq).total; }
+size_t mv_lds_bytes(int nq) { return sizeof(double) * MvLayout<NUP, R>(nq).total; }
 
-// FP64 rebuild of the inverse (and phases); FP32 copies for the mixed variant
-template <int NUP, typename R>
-__device__ bool sm_refresh(const DevSys &S, double *sm, const SmLayout<NUP, R> &X,
-                           const Tile<NUP> &T, int tid) {
-    const DetInfo D = build_inverse<NUP>(S, sm, X.L, T, tid);
-    if constexpr (sizeof(R) == 4) {
-        constexpr int SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
-        float *sv = reinterpret_cast<float *>(sm + X.sv);
-        const double *s64 = sm + X.L.sinv;
-        for (int e = tid; e < NUP * NUP; e += 64) {
-            const int j = e / NUP, i = e - j * NUP;
-            sv[j * SP + i] = float(s64[j * SP + i]);
+// exp(i q.r) of one position for one q, R-typed
+template <typename R>
+__device__ __forceinline__ typename Vec2<R>::T phase_q(const DevSys &S, const int4 m,
+                                                       const double *pos) {
+    R c[3], s[3];
+    const int mm[3] = {m.x, m.y, m.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        R s1, c1;
+        if constexpr (sizeof(R) == 8) {
+            sincos(pos[d] * S.twopi_l, &s1, &c1);
+        } else {
+            sincosf(float(pos[d] * S.twopi_l), &s1, &c1);
         }
-        float2 *ph = reinterpret_cast<float2 *>(sm + X.ph);
-        const double2 *p64 = reinterpret_cast<const double2 *>(sm + X.L.work);
-        for (int e = tid; e < NP * S.nq; e += 64) ph[e] = make_float2(float(p64[e].x), float(p64[e].y));
-        __syncthreads();
+        // exp(i k theta) for k = |m_d| <= 4 by repeated multiplication
+        const int am = mm[d] < 0 ? -mm[d] : mm[d];
+        R cc = R(1), ss = R(0);
+#pragma unroll
+        for (int k = 1; k < 5; ++k) {
+            const R nc = cc * c1 - ss * s1, ns = ss * c1 + cc * s1;
+            const bool t = k <= am;
+            cc = t ? nc : cc;
+            ss = t ? ns : ss;
+        }
+        c[d] = cc;
+        s[d] = mm[d] < 0 ? -ss : ss;
     }
-    return D.singular;
+    const R cxy = c[0] * c[1] - s[0] * s[1], sxy = s[0] * c[1] + c[0] * s[1];
+    typename Vec2<R>::T z;
+    z.x = cxy * c[2] - sxy * s[2];
+    z.y = sxy * c[2] + cxy * s[2];
+    return z;
 }
 
 template <int NUP, typename R>
-__global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B, DevStep T,
-                                                int kref, int opcomp) {
+__global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restrict__ xin,
+                                                 double *xout, const uint64_t *irn_in,
+                                                 uint64_t *irn_out, char *sinv_g, int m0,
+                                                 int m1, DevStep T) {
     using R2 = typename Vec2<R>::T;
     extern __shared__ double sm[];
-    const SmLayout<NUP, R> X(S.nq);
-    const Layout<NUP> &L = X.L;
-    constexpr int M = 6 * NUP, NP = 2 * NUP, SP = Layout<NUP>::SP;
+    const MvLayout<NUP, R> X(S.nq);
+    constexpr int M = 6 * NUP, NP = 2 * NUP, SP = MvLayout<NUP, R>::SP;
     const int w = blockIdx.x, tid = threadIdx.x;
     if (w >= T.nt[0]) return;
     const int nq = S.nq;
-    load_q<NUP>(S, sm, L, tid);
-    if constexpr (sizeof(R) == 4) {
-        float *q32 = reinterpret_cast<float *>(sm + X.qc);
-        __syncthreads();
-        for (int e = tid; e < 5 * nq; e += 64) q32[e] = float(sm[L.qc + e]);
-    }
-    double *xs = sm + L.xs;
-    for (int t = tid; t < M; t += 64) xs[t] = wrapd(A.x[size_t(w) * M + t], S);
-    __syncthreads();
-    const Tile<NUP> Tl(tid);
-    bool singular = sm_refresh<NUP, R>(S, sm, X, Tl, tid);
-
+    double *xs = sm + X.xs;
+    R *qc = reinterpret_cast<R *>(sm + X.qc);
+    int4 *qm = reinterpret_cast<int4 *>(sm + X.qm);
     R *sv = reinterpret_cast<R *>(sm + X.sv);
     R2 *ph = reinterpret_cast<R2 *>(sm + X.ph);
-    const R *qc = reinterpret_cast<const R *>(sm + X.qc);
-    R2 *phn = reinterpret_cast<R2 *>(sm + L.xn + 4);
-    R *phinew = reinterpret_cast<R *>(sm + L.prow);
-    R *colbuf = reinterpret_cast<R *>(sm + L.mcol);
-    R *fbuf = reinterpret_cast<R *>(sm + L.dp);
-    const R alpha0 = R(S.alpha0);
-    const double sig = S.sigma, sig2 = S.sig2;
-    uint64_t s = A.irn[w];
-    int nacc = 0;
+    R2 *phn = reinterpret_cast<R2 *>(sm + X.phn);
+    R *phinew = reinterpret_cast<R *>(sm + X.buf);
+    R *colbuf = phinew + SP;
+    R *fbuf = colbuf + SP;
 
-    for (int ip = 0; ip < NP; ++ip) {
-        if (ip > 0 && ip % kref == 0) singular = sm_refresh<NUP, R>(S, sm, X, Tl, tid);
+    for (int q = tid; q < nq; q += 64) {
+        const int4 m = S.qm[q];
+        const double wq = S.wq[q], tp = S.twopi_l;
+        qm[q] = m;
+        qc[5 * q] = R(wq);
+        qc[5 * q + 1] = R(-wq * tp * m.x);
+        qc[5 * q + 2] = R(-wq * tp * m.y);
+        qc[5 * q + 3] = R(-wq * tp * m.z);
+    }
+    for (int t = tid; t < M; t += 64) xs[t] = wrapd(xin[size_t(w) * M + t], S);
+    {
+        const R *g = reinterpret_cast<const R *>(sinv_g + size_t(w) * sinv_bytes(S));
+        for (int e = tid; e < NUP * NUP; e += 64) {
+            const int j = e / NUP, i = e - j * NUP;
+            sv[j * SP + i] = g[e];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < NP * nq; e += 64) {
+        const int p = e / nq, q = e - p * nq;
+        ph[e] = phase_q<R>(S, qm[q], xs + 3 * p);
+    }
+    __syncthreads();
+
+    const double sig = S.sigma, sig2 = S.sig2;
+    const R alpha0 = R(S.alpha0);
+    uint64_t s = irn_in[w];
+    int nacc = 0;
+    for (int ip = m0; ip < m1; ++ip) {
         const bool up = ip < NUP;
         const int a = up ? ip : ip - NUP;
         const double xa[3] = {xs[3 * ip], xs[3 * ip + 1], xs[3 * ip + 2]};
-        // old gradient of particle ip: lanes b < NUP over the opposite-spin partners
+        // old gradient: lanes b < NUP over the opposite-spin partners
         R go0 = 0, go1 = 0, go2 = 0, jo0 = 0, jo1 = 0, jo2 = 0, uo = 0, wgt = 0;
         int pb = 0;
         if (tid < NUP) {
@@ -175,8 +207,8 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
                 go2 += qc[5 * q + 3] * sab;
             }
             if (!S.nobeta && ro <= R(S.rangeb)) {
-                R bv, db, lb, pad;
-                tab_at<R>(S.bt, S.bt32, R(S.scaleb) * ro, S.ntab, bv, db, lb, pad);
+                R bv, db;
+                tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * ro, S.ntab, bv, db);
                 const R t = db / ro;
                 go0 += t * R(dx[0]);
                 go1 += t * R(dx[1]);
@@ -187,8 +219,8 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
             go2 *= wgt;
             const R rj = ro > R(1.0e-14) ? ro : R(1.0e-14);
             if (rj < R(S.range)) {
-                R vv, uu, du, d2u;
-                tab_at<R>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, vv, uu, du, d2u);
+                R uu, du;
+                tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
                 uo = uu;
                 const R t = du / rj;
                 jo0 = R(dx[0]) * t;
@@ -198,7 +230,7 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
         }
         const double go[3] = {double(wsum(go0)), double(wsum(go1)), double(wsum(go2))};
         const double jo[3] = {double(wsum(jo0)), double(wsum(jo1)), double(wsum(jo2))};
-        // proposal: gaussian(3) then the acceptance draw (5 ran1 draws per move)
+        // gaussian(3) then the acceptance draw: five ran1 draws per move
         double u[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
@@ -228,36 +260,7 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
             dxm[d] = dro[d] + sig * g3[d];
             xn[d] = wrapd(xa[d] + dxm[d], S);
         }
-        // phases of the proposed position: lanes over q
-        if (tid < nq) {
-            double pc[3][5], ps[3][5];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                if constexpr (sizeof(R) == 8) {
-                    sincos(xn[d] * S.twopi_l, &ps[d][1], &pc[d][1]);
-                } else {
-                    float sf, cf;
-                    sincosf(float(xn[d] * S.twopi_l), &sf, &cf);
-                    ps[d][1] = sf;
-                    pc[d][1] = cf;
-                }
-                pc[d][0] = 1.0;
-                ps[d][0] = 0.0;
-#pragma unroll
-                for (int m = 2; m < 5; ++m) {
-                    pc[d][m] = pc[d][m - 1] * pc[d][1] - ps[d][m - 1] * ps[d][1];
-                    ps[d][m] = ps[d][m - 1] * pc[d][1] + pc[d][m - 1] * ps[d][1];
-                }
-            }
-            const int4 m = reinterpret_cast<const int4 *>(sm + L.ints + 64)[tid];
-            double cx, sx, cy, sy, cz, sz;
-            cpow(pc[0], ps[0], m.x, cx, sx);
-            cpow(pc[1], ps[1], m.y, cy, sy);
-            cpow(pc[2], ps[2], m.z, cz, sz);
-            const double cxy = cx * cy - sx * sy, sxy = sx * cy + cx * sy;
-            phn[tid].x = R(cxy * cz - sxy * sz);
-            phn[tid].y = R(sxy * cz + cxy * sz);
-        }
+        if (tid < nq) phn[tid] = phase_q<R>(S, qm[tid], xn);
         __syncthreads();
         // pair values and gradients at the proposed position
         double rho_part = 0.0;
@@ -283,8 +286,8 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
             }
             f += alpha0;
             if (!S.nobeta && rn <= R(S.rangeb)) {
-                R bv, db, lb, pad;
-                tab_at<R>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db, lb, pad);
+                R bv, db;
+                tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db);
                 f += bv;
                 const R t = db / rn;
                 g0 += t * R(dx[0]);
@@ -299,8 +302,8 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
             const R rj = rn > R(1.0e-14) ? rn : R(1.0e-14);
             R un = 0;
             if (rj < R(S.range)) {
-                R vv, uu, du, d2u;
-                tab_at<R>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, vv, uu, du, d2u);
+                R uu, du;
+                tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
                 un = uu;
                 const R t = du / rj;
                 jn0 = R(dx[0]) * t;
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
         const double gn[3] = {double(wsum(gn0)), double(wsum(gn1)), double(wsum(gn2))};
         const double jn[3] = {double(wsum(jn0)), double(wsum(jn1)), double(wsum(jn2))};
         const double dlogj = double(wsum(dU));
-        bool accept = !singular && !(S.fn && !(rho > 0.0)) && rho != 0.0;
+        bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
             double logp = 2.0 * (log(fabs(rho)) + dlogj);
 #pragma unroll
@@ -350,39 +353,119 @@ __global__ __launch_bounds__(64) void k_sm_step(DevSys S, DevState A, DevState B
         }
         __syncthreads();
     }
-    // end of sweep: FP64 hpsi at the final positions, weights, estimators, branch draw
-    hpsi_block<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) xout[size_t(w) * M + t] = xs[t];
     if (tid == 0) {
-        T.nacc[w] = nacc;
+        irn_out[w] = s;
+        T.nacc[w] = (m0 == 0 ? 0 : T.nacc[w]) + nacc;
         atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[0]), (unsigned long long)nacc);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[1]), (unsigned long long)NP);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[1]),
+                  (unsigned long long)(m1 - m0));
     }
-    finish_walker<NUP>(S, sm, L, A, B, T, w, tid, s, opcomp);
+}
+
+// FP64 rebuild of the inverse of positions x (every krefresh moves)
+template <int NUP>
+__global__ __launch_bounds__(64) void k_sm_refresh(DevSys S, const double *__restrict__ x,
+                                                   char *sinv_g, DevStep T) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= T.nt[0]) return;
+    constexpr int M = 6 * NUP;
+    load_q<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
+    __syncthreads();
+    const Tile<NUP> Tl(tid);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid);
+    store_sinv<NUP>(S, sm, L, D.singular, sinv_g + size_t(w) * sinv_bytes(S), tid);
+}
+
+// end of the sweep: FP64 hpsi of the final positions B.x, the fresh inverse into B.sinv,
+// weight against the energy of A, estimators and branch draw (state after the moves in
+// B.irn)
+template <int NUP>
+__global__ __launch_bounds__(64) void k_sm_end(DevSys S, DevState A, DevState B, DevStep T,
+                                               int opcomp) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= T.nt[0]) return;
+    constexpr int M = 6 * NUP;
+    load_q<NUP>(S, sm, L, tid);
+    for (int t = tid; t < M; t += 64) sm[L.xs + t] = B.x[size_t(w) * M + t];
+    __syncthreads();
+    const Tile<NUP> Tl(tid);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid);
+    store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_bytes(S), tid);
+    grad_lap<NUP>(S, sm, L, Tl, D, tid);
+    finish_walker<NUP>(S, sm, L, A, B, T, w, tid, B.irn[w], opcomp);
 }
 
 template <int NUP>
-int sm_launch(int mixed, const DevSys &S, const DevState &A, const DevState &B,
-              const DevStep &T, int krefresh, int opcomp, int grid, hipStream_t st) {
-    if (mixed) {
-        const size_t lds = sm_lds_bytes<NUP, float>(S.nq);
-        hipLaunchKernelGGL((k_sm_step<NUP, float>), dim3(grid), dim3(64), lds, st, S, A, B, T,
-                           krefresh, opcomp);
+int sm_moves_t(const DevSys &S, const double *xin, double *xout, const uint64_t *irn_in,
+               uint64_t *irn_out, char *sinv, int m0, int m1, const DevStep &T, int grid,
+               hipStream_t st) {
+    if (S.sinv_f32) {
+        const size_t lds = mv_lds_bytes<NUP, float>(S.nq);
+        hipLaunchKernelGGL((k_sm_moves<NUP, float>), dim3(grid), dim3(64), lds, st, S, xin, xout,
+                           irn_in, irn_out, sinv, m0, m1, T);
     } else {
-        const size_t lds = sm_lds_bytes<NUP, double>(S.nq);
-        hipLaunchKernelGGL((k_sm_step<NUP, double>), dim3(grid), dim3(64), lds, st, S, A, B, T,
-                           krefresh, opcomp);
+        const size_t lds = mv_lds_bytes<NUP, double>(S.nq);
+        hipLaunchKernelGGL((k_sm_moves<NUP, double>), dim3(grid), dim3(64), lds, st, S, xin,
+                           xout, irn_in, irn_out, sinv, m0, m1, T);
     }
     return hipGetLastError();
 }
 
-int launch_sm_step(int nup, int mixed, const DevSys &S, const DevState &A, const DevState &B,
-                   const DevStep &T, int krefresh, int opcomp, int grid, hipStream_t st) {
+template <int NUP>
+int sm_refresh_t(const DevSys &S, const double *x, char *sinv, const DevStep &T, int grid,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_sm_refresh<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq), st, S, x,
+                       sinv, T);
+    return hipGetLastError();
+}
+
+template <int NUP>
+int sm_end_t(const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
+             int opcomp, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_sm_end<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq), st, S, A, B, T,
+                       opcomp);
+    return hipGetLastError();
+}
+
+int launch_sm_moves(int nup, const DevSys &S, const double *xin, double *xout,
+                    const uint64_t *irn_in, uint64_t *irn_out, char *sinv, int m0, int m1,
+                    const DevStep &T, int grid, hipStream_t st) {
     if (grid <= 0) return hipSuccess;
     switch (nup) {
-        case 1: return sm_launch<1>(mixed, S, A, B, T, krefresh, opcomp, grid, st);
-        case 7: return sm_launch<7>(mixed, S, A, B, T, krefresh, opcomp, grid, st);
-        case 19: return sm_launch<19>(mixed, S, A, B, T, krefresh, opcomp, grid, st);
-        case 33: return sm_launch<33>(mixed, S, A, B, T, krefresh, opcomp, grid, st);
+        case 1: return sm_moves_t<1>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
+        case 7: return sm_moves_t<7>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
+        case 19: return sm_moves_t<19>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
+        case 33: return sm_moves_t<33>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+int launch_sm_refresh(int nup, const DevSys &S, const double *x, char *sinv, const DevStep &T,
+                      int grid, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
+    switch (nup) {
+        case 1: return sm_refresh_t<1>(S, x, sinv, T, grid, st);
+        case 7: return sm_refresh_t<7>(S, x, sinv, T, grid, st);
+        case 19: return sm_refresh_t<19>(S, x, sinv, T, grid, st);
+        case 33: return sm_refresh_t<33>(S, x, sinv, T, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+int launch_sm_end(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                  const DevStep &T, int opcomp, int grid, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
+    switch (nup) {
+        case 1: return sm_end_t<1>(S, A, B, T, opcomp, grid, st);
+        case 7: return sm_end_t<7>(S, A, B, T, opcomp, grid, st);
+        case 19: return sm_end_t<19>(S, A, B, T, opcomp, grid, st);
+        case 33: return sm_end_t<33>(S, A, B, T, opcomp, grid, st);
     }
     return hipErrorInvalidValue;
 }

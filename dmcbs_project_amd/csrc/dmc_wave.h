@@ -502,6 +502,32 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
     __syncthreads();
 }
 
+// copy the inverse left in LDS by build_inverse to global memory, dense [j][i], as FP64
+// or FP32 (zeros for a singular matrix)
+template <int NUP>
+__device__ void store_sinv(const DevSys &S, const double *sm, const Layout<NUP> &L, bool singular,
+                           char *dst, int tid) {
+    constexpr int SP = Layout<NUP>::SP;
+    const double *sv = sm + L.sinv;
+    if (S.sinv_f32) {
+        float *d = reinterpret_cast<float *>(dst);
+        for (int e = tid; e < NUP * NUP; e += 64) {
+            const int j = e / NUP, i = e - j * NUP;
+            d[e] = singular ? 0.0f : float(sv[j * SP + i]);
+        }
+    } else {
+        double *d = reinterpret_cast<double *>(dst);
+        for (int e = tid; e < NUP * NUP; e += 64) {
+            const int j = e / NUP, i = e - j * NUP;
+            d[e] = singular ? 0.0 : sv[j * SP + i];
+        }
+    }
+}
+
+__host__ __device__ inline size_t sinv_bytes(const DevSys &S) {
+    return size_t(S.n) * S.n * (S.sinv_f32 ? 4 : 8);
+}
+
 template <int NUP>
 __device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
                                            int tid) {
