@@ -116,17 +116,16 @@ __global__ __launch_bounds__(64) void k_ratio(DevSys S, const double *__restrict
     if (tid < NUP) {
         const int j = tid;
         if (i < NUP) {
-            const double f = phi_pair(S, qc, phn, ph + (NUP + j) * S.nq, xn, xs + 3 * (NUP + j));
+            const double f = phi_pair(S, qc, phn, 1, ph + NUP + j, NP, xn, xs + 3 * (NUP + j));
             part = sinv[j * SP + i] * f;
         } else {
-            const double f = phi_pair(S, qc, ph + j * S.nq, phn, xs + 3 * j, xn);
+            const double f = phi_pair(S, qc, ph + j, NP, phn, 1, xs + 3 * j, xn);
             part = sinv[(i - NUP) * SP + j] * f;
         }
     }
     const double rdet = wave_sum(part);
     const double lj = jastrow_logratio<NUP>(S, xs, i, xn, tid);
     if (tid == 0) ratio[w] = D.singular ? 0.0 : exp(lj) * rdet;
-    (void)NP;
 }
 
 // ------------------------------------------------------------------------------------

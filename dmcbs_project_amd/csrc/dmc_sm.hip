@@ -38,10 +38,33 @@ struct Vec2<float> {
     using T = float2;
 };
 
-template <typename R>
-__device__ __forceinline__ R wsum(R v) {
+// lane exchanges inside a 16-lane row by DPP (xor 1, xor 2, mirror within 8, mirror within
+// 16), then across rows by shuffles: the sum of all 64 lanes in every lane
+__device__ __forceinline__ int dpp_i(int v, int ctrl) {
+    switch (ctrl) {
+        case 0: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+        default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror
+    }
+}
+__device__ __forceinline__ float wsum(float v) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    for (int k = 0; k < 4; ++k) v += __int_as_float(dpp_i(__float_as_int(v), k));
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long b = __double_as_longlong(v);
+        const int lo = dpp_i(int(b & 0xffffffffll), k), hi = dpp_i(int(b >> 32), k);
+        v += __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                  static_cast<unsigned int>(lo));
+    }
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -78,9 +101,11 @@ __device__ __forceinline__ void tab2(const double4 *t64, const float4 *t32, R s,
 // LDS of the move kernel (offsets in doubles)
 template <int NUP, typename R>
 struct MvLayout {
-    static constexpr int SP = NUP + 1, NP = 2 * NUP;
+    // row stride of the inverse: >= NUP and = 1 (mod 16), so that lanes walking a column of
+    // the inverse hit distinct banks
+    static constexpr int SP = ((NUP + 14) / 16) * 16 + 1, NP = 2 * NUP;
     static constexpr int RD = sizeof(R) == 8 ? 1 : 2;  // R values per double
-    int xs, qc, qm, sv, ph, phn, buf, total;
+    int xs, qc, qm, sv, ph, phn, pw, buf, total;
     __host__ __device__ explicit MvLayout(int nq) {
         xs = 0;
         qc = xs + 3 * NP;
@@ -88,7 +113,8 @@ struct MvLayout {
         sv = qm + 2 * nq;
         ph = sv + (NUP * SP + RD - 1) / RD;
         phn = ph + (2 * NP * nq + RD - 1) / RD;
-        buf = phn + (2 * nq + RD - 1) / RD;
+        pw = phn + (2 * nq + RD - 1) / RD;      // exp(i m theta_d), d < 3, m = 0..4
+        buf = pw + (2 * 15 + RD - 1) / RD;
         total = buf + (3 * SP + RD - 1) / RD;
     }
 };
@@ -96,32 +122,38 @@ struct MvLayout {
 template <int NUP, typename R>
 size_t mv_lds_bytes(int nq) { return sizeof(double) * MvLayout<NUP, R>(nq).total; }
 
-// exp(i q.r) of one position for one q, R-typed
+// exp(i m theta_d) for m = 0..4 of one coordinate, R-typed
 template <typename R>
-__device__ __forceinline__ typename Vec2<R>::T phase_q(const DevSys &S, const int4 m,
-                                                       const double *pos) {
-    R c[3], s[3];
+__device__ __forceinline__ void powers(const DevSys &S, double x, R (&c)[5], R (&s)[5]) {
+    R s1, c1;
+    if constexpr (sizeof(R) == 8) {
+        sincos(x * S.twopi_l, &s1, &c1);
+    } else {
+        sincosf(float(x * S.twopi_l), &s1, &c1);
+    }
+    c[0] = R(1);
+    s[0] = R(0);
+    c[1] = c1;
+    s[1] = s1;
+#pragma unroll
+    for (int k = 2; k < 5; ++k) {
+        c[k] = c[k - 1] * c1 - s[k - 1] * s1;
+        s[k] = s[k - 1] * c1 + c[k - 1] * s1;
+    }
+}
+
+// product of the three factors exp(i m_d theta_d), read from a power table pw[d][m]
+template <typename R>
+__device__ __forceinline__ typename Vec2<R>::T qphase(const typename Vec2<R>::T *pw,
+                                                      const int4 m) {
     const int mm[3] = {m.x, m.y, m.z};
+    R c[3], s[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        R s1, c1;
-        if constexpr (sizeof(R) == 8) {
-            sincos(pos[d] * S.twopi_l, &s1, &c1);
-        } else {
-            sincosf(float(pos[d] * S.twopi_l), &s1, &c1);
-        }
-        // exp(i k theta) for k = |m_d| <= 4 by repeated multiplication
         const int am = mm[d] < 0 ? -mm[d] : mm[d];
-        R cc = R(1), ss = R(0);
-#pragma unroll
-        for (int k = 1; k < 5; ++k) {
-            const R nc = cc * c1 - ss * s1, ns = ss * c1 + cc * s1;
-            const bool t = k <= am;
-            cc = t ? nc : cc;
-            ss = t ? ns : ss;
-        }
-        c[d] = cc;
-        s[d] = mm[d] < 0 ? -ss : ss;
+        const typename Vec2<R>::T v = pw[5 * d + am];
+        c[d] = v.x;
+        s[d] = mm[d] < 0 ? -v.y : v.y;
     }
     const R cxy = c[0] * c[1] - s[0] * s[1], sxy = s[0] * c[1] + c[0] * s[1];
     typename Vec2<R>::T z;
@@ -148,6 +180,7 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
     R *sv = reinterpret_cast<R *>(sm + X.sv);
     R2 *ph = reinterpret_cast<R2 *>(sm + X.ph);
     R2 *phn = reinterpret_cast<R2 *>(sm + X.phn);
+    R2 *pw = reinterpret_cast<R2 *>(sm + X.pw);
     R *phinew = reinterpret_cast<R *>(sm + X.buf);
     R *colbuf = phinew + SP;
     R *fbuf = colbuf + SP;
@@ -170,9 +203,36 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
         }
     }
     __syncthreads();
-    for (int e = tid; e < NP * nq; e += 64) {
-        const int p = e / nq, q = e - p * nq;
-        ph[e] = phase_q<R>(S, qm[q], xs + 3 * p);
+    // q-major phases ph[q * NP + p] of every particle (lanes over particles)
+    for (int p = tid; p < NP; p += 64) {
+        R c[3][5], sn[3][5];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) powers<R>(S, xs[3 * p + d], c[d], sn[d]);
+        for (int q = 0; q < nq; ++q) {
+            const int4 m = qm[q];
+            const int mm[3] = {m.x, m.y, m.z};
+            R cq[3], sq[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                // select kept apart from an indexed private load (no scratch)
+                const int am = mm[d] < 0 ? -mm[d] : mm[d];
+                R cc = c[d][0], ss = sn[d][0];
+#pragma unroll
+                for (int k = 1; k < 5; ++k) {
+                    const bool t = am == k;
+                    cc = t ? c[d][k] : cc;
+                    ss = t ? sn[d][k] : ss;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                cq[d] = cc;
+                sq[d] = mm[d] < 0 ? -ss : ss;
+            }
+            const R cxy = cq[0] * cq[1] - sq[0] * sq[1], sxy = sq[0] * cq[1] + cq[0] * sq[1];
+            R2 z;
+            z.x = cxy * cq[2] - sxy * sq[2];
+            z.y = sxy * cq[2] + cxy * sq[2];
+            ph[q * NP + p] = z;
+        }
     }
     __syncthreads();
 
@@ -185,7 +245,7 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
         const int a = up ? ip : ip - NUP;
         const double xa[3] = {xs[3 * ip], xs[3 * ip + 1], xs[3 * ip + 2]};
         // old gradient: lanes b < NUP over the opposite-spin partners
-        R go0 = 0, go1 = 0, go2 = 0, jo0 = 0, jo1 = 0, jo2 = 0, uo = 0, wgt = 0;
+        R go0 = 0, go1 = 0, go2 = 0, uo = 0, wgt = 0;
         int pb = 0;
         if (tid < NUP) {
             const int b = tid;
@@ -198,38 +258,27 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 r2 += dx[d] * dx[d];
             }
             const R ro = R(sqrt(r2));
-            const R2 *za = ph + ip * nq, *zb = ph + pb * nq;
+            // table values first, so that their loads overlap the plane-wave sum
+            R bv = 0, db = 0, uu = 0, du = 0;
+            const R rj = ro > R(1.0e-14) ? ro : R(1.0e-14);
+            if (!S.nobeta && ro <= R(S.rangeb)) tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * ro, S.ntab, bv, db);
+            if (rj < R(S.range)) tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
+#pragma unroll 4
             for (int q = 0; q < nq; ++q) {
-                const R2 u = za[q], v = zb[q];
+                const R2 u = ph[q * NP + ip], v = ph[q * NP + pb];
                 const R sab = u.y * v.x - u.x * v.y;
                 go0 += qc[5 * q + 1] * sab;
                 go1 += qc[5 * q + 2] * sab;
                 go2 += qc[5 * q + 3] * sab;
             }
-            if (!S.nobeta && ro <= R(S.rangeb)) {
-                R bv, db;
-                tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * ro, S.ntab, bv, db);
-                const R t = db / ro;
-                go0 += t * R(dx[0]);
-                go1 += t * R(dx[1]);
-                go2 += t * R(dx[2]);
-            }
-            go0 *= wgt;
-            go1 *= wgt;
-            go2 *= wgt;
-            const R rj = ro > R(1.0e-14) ? ro : R(1.0e-14);
-            if (rj < R(S.range)) {
-                R uu, du;
-                tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
-                uo = uu;
-                const R t = du / rj;
-                jo0 = R(dx[0]) * t;
-                jo1 = R(dx[1]) * t;
-                jo2 = R(dx[2]) * t;
-            }
+            const R t = db / ro, tj = du / rj;
+            uo = uu;
+            // determinant part weighted by the inverse, plus the Jastrow part
+            go0 = wgt * (go0 + t * R(dx[0])) + R(dx[0]) * tj;
+            go1 = wgt * (go1 + t * R(dx[1])) + R(dx[1]) * tj;
+            go2 = wgt * (go2 + t * R(dx[2])) + R(dx[2]) * tj;
         }
         const double go[3] = {double(wsum(go0)), double(wsum(go1)), double(wsum(go2))};
-        const double jo[3] = {double(wsum(jo0)), double(wsum(jo1)), double(wsum(jo2))};
         // gaussian(3) then the acceptance draw: five ran1 draws per move
         double u[5];
 #pragma unroll
@@ -255,12 +304,23 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
         double dro[3], dxm[3], xn[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            dro[d] = (go[d] + jo[d]) * sig2;
+            dro[d] = go[d] * sig2;
             if (S.cutdrift) dro[d] = fmin(S.dcut, fmax(-S.dcut, dro[d]));
             dxm[d] = dro[d] + sig * g3[d];
             xn[d] = wrapd(xa[d] + dxm[d], S);
         }
-        if (tid < nq) phn[tid] = phase_q<R>(S, qm[tid], xn);
+        // phases of the proposed position: powers per axis (lanes 0..2), then lanes over q
+        if (tid < 3) {
+            R c[5], sn[5];
+            powers<R>(S, xn[tid], c, sn);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                pw[5 * tid + k].x = c[k];
+                pw[5 * tid + k].y = sn[k];
+            }
+        }
+        __syncthreads();
+        if (tid < nq) phn[tid] = qphase<R>(pw, qm[tid]);
         __syncthreads();
         // pair values and gradients at the proposed position
         double rho_part = 0.0;
@@ -273,10 +333,14 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 r2 += dx[d] * dx[d];
             }
             const R rn = R(sqrt(r2));
-            const R2 *zb = ph + pb * nq;
+            R bv = 0, db = 0, uu = 0, du = 0;
+            const R rj = rn > R(1.0e-14) ? rn : R(1.0e-14);
+            if (!S.nobeta && rn <= R(S.rangeb)) tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db);
+            if (rj < R(S.range)) tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
             R f = 0, g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll 4
             for (int q = 0; q < nq; ++q) {
-                const R2 un = phn[q], v = zb[q];
+                const R2 un = phn[q], v = ph[q * NP + pb];
                 const R cab = un.x * v.x + un.y * v.y;
                 const R sab = un.y * v.x - un.x * v.y;
                 f += qc[5 * q] * cab;
@@ -284,33 +348,21 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 g1 += qc[5 * q + 2] * sab;
                 g2 += qc[5 * q + 3] * sab;
             }
-            f += alpha0;
-            if (!S.nobeta && rn <= R(S.rangeb)) {
-                R bv, db;
-                tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db);
-                f += bv;
-                const R t = db / rn;
-                g0 += t * R(dx[0]);
-                g1 += t * R(dx[1]);
-                g2 += t * R(dx[2]);
-            }
+            f += alpha0 + bv;
+            const R t = db / rn;
+            g0 += t * R(dx[0]);
+            g1 += t * R(dx[1]);
+            g2 += t * R(dx[2]);
             phinew[tid] = f;
             rho_part = double(wgt) * double(f);
             gn0 = wgt * g0;
             gn1 = wgt * g1;
             gn2 = wgt * g2;
-            const R rj = rn > R(1.0e-14) ? rn : R(1.0e-14);
-            R un = 0;
-            if (rj < R(S.range)) {
-                R uu, du;
-                tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
-                un = uu;
-                const R t = du / rj;
-                jn0 = R(dx[0]) * t;
-                jn1 = R(dx[1]) * t;
-                jn2 = R(dx[2]) * t;
-            }
-            dU = un - uo;
+            const R tj = du / rj;
+            jn0 = R(dx[0]) * tj;
+            jn1 = R(dx[1]) * tj;
+            jn2 = R(dx[2]) * tj;
+            dU = uu - uo;
         }
         const double rho = wave_sum(rho_part);  // FP64 sum: the node test uses its sign
         const double gn[3] = {double(wsum(gn0)), double(wsum(gn1)), double(wsum(gn2))};
@@ -337,19 +389,25 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 const int k = tid;
                 R z = 0;
                 if (up) {
+#pragma unroll
                     for (int b = 0; b < NUP; ++b) z += phinew[b] * sv[b * SP + k];
                 } else {
+#pragma unroll
                     for (int b = 0; b < NUP; ++b) z += sv[k * SP + b] * phinew[b];
                 }
                 fbuf[k] = (z - (k == a ? R(1) : R(0))) / rr;
             }
             __syncthreads();
-            for (int e = tid; e < NUP * NUP; e += 64) {
-                const int r = e / NUP, c = e - r * NUP;
-                sv[r * SP + c] -= up ? colbuf[r] * fbuf[c] : fbuf[r] * colbuf[c];
+#pragma unroll
+            for (int e0 = 0; e0 < NUP * NUP; e0 += 64) {
+                const int e = e0 + tid;
+                if (e < NUP * NUP) {
+                    const int r = e / NUP, c = e - r * NUP;
+                    sv[r * SP + c] -= up ? colbuf[r] * fbuf[c] : fbuf[r] * colbuf[c];
+                }
             }
             if (tid < 3) xs[3 * ip + tid] = xn[tid];
-            if (tid < nq) ph[ip * nq + tid] = phn[tid];
+            if (tid < nq) ph[tid * NP + ip] = phn[tid];
         }
         __syncthreads();
     }

@@ -191,7 +191,8 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     const bool *rv = T.rv, *cv = T.cv;
 
     // Phase A: plane-wave phases of every particle
-    for (int p = tid; p < NP; p += 64) phases_of(S, qm, xs + 3 * p, ph + p * nq, 1);
+    // q-major: ph[q * NP + p], so that lanes over particles read consecutive words
+    for (int p = tid; p < NP; p += 64) phases_of(S, qm, xs + 3 * p, ph + p, NP);
     __syncthreads();
 
     // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij)
@@ -214,10 +215,10 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
         double2 zi[TR], zj[TR];
 #pragma unroll
         for (int a = 0; a < TR; ++a) {
-            zi[a] = ph[ri[a] * nq + q];
+            zi[a] = ph[q * NP + ri[a]];
             zi[a].x *= w;
             zi[a].y *= w;
-            zj[a] = ph[(NUP + cj[a]) * nq + q];
+            zj[a] = ph[q * NP + NUP + cj[a]];
         }
 #pragma unroll
         for (int a = 0; a < TR; ++a)
@@ -412,8 +413,8 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
         double2 zi[TR], zj[TR];
 #pragma unroll
         for (int a = 0; a < TR; ++a) {
-            zi[a] = ph[ri[a] * nq + q];
-            zj[a] = ph[(NUP + cj[a]) * nq + q];
+            zi[a] = ph[q * NP + ri[a]];
+            zj[a] = ph[q * NP + NUP + cj[a]];
         }
         double X[TR], Y[TR], Z[TR], W[TR];
 #pragma unroll
@@ -567,7 +568,7 @@ __device__ double jastrow_logratio(const DevSys &S, const double *xs, int i, con
 
 // pair function phi(xa - xb) from the phases of both positions (allpair element)
 __device__ __forceinline__ double phi_pair(const DevSys &S, const double *qc, const double2 *za,
-                                           const double2 *zb, const double *xa,
+                                           int sa, const double2 *zb, int sb, const double *xa,
                                            const double *xb) {
     double r2 = 0.0;
 #pragma unroll
@@ -576,7 +577,8 @@ __device__ __forceinline__ double phi_pair(const DevSys &S, const double *qc, co
         r2 += dx * dx;
     }
     double f = S.alpha0 + beta_val(S, sqrt(r2));
-    for (int q = 0; q < S.nq; ++q) f += qc[5 * q] * (za[q].x * zb[q].x + za[q].y * zb[q].y);
+    for (int q = 0; q < S.nq; ++q)
+        f += qc[5 * q] * (za[q * sa].x * zb[q * sb].x + za[q * sa].y * zb[q * sb].y);
     return f;
 }
 
