@@ -32,14 +32,24 @@ N, NORB, NWALK, DT, SEED = 66, 5, 32768, 0.001, 7
 DCUT_SM = 2.0
 
 
-def flops_per_walker_step(mode, n=N // 2, k=N, kq=17):
-    """Algorithmic flop counts per walker-step (C = 2K - 1 plane-wave columns)."""
+def flops_per_walker_step(mode, n=N // 2, k=N, kq=17, part="step"):
+    """Algorithmic flop counts per walker-step (C = 2K - 1 plane-wave columns).
+
+    all-particle: F_AP = 10 n^2 C + 2 n^3 + 16 n^2 + 70 n^2 + 90 n^2 (622,908 at N=66);
+    single-particle sweep: F_SM(k) = N(8nC + 4n^2 + 164n) [the moves]
+                                     + (N/k)(2n^3 + 2n^2 C) [refresh]
+                                     + N(2nC + 2n) + 16 n^2 [local energy]
+    (1,528,956 at N=66, k=66); part="moves" gives the first term alone (1,219,680), the
+    work of the move kernel."""
     c = 2 * kq - 1
     if mode == "allparticle":
         return 10 * n * n * c + 2 * n ** 3 + 16 * n * n + 70 * n * n + 90 * n * n
     nn = 2 * n
-    return (nn * (8 * n * c + 4 * n * n + 164 * n) + (nn / k) * (2 * n ** 3 + 2 * n * n * c)
-            + nn * (2 * n * c + 2 * n) + 16 * n * n)
+    moves = nn * (8 * n * c + 4 * n * n + 164 * n)
+    if part == "moves":
+        return moves
+    return (moves + (nn / k) * (2 * n ** 3 + 2 * n * n * c) + nn * (2 * n * c + 2 * n)
+            + 16 * n * n)
 
 
 def make_engine(mode, nwalk=NWALK, krefresh=N):
@@ -119,6 +129,36 @@ def elocal_error(nconf=4096):
                 walkers=int(len(w0["irn"])))
 
 
+def traffic_from_profiles(workload):
+    """HBM bytes per launch of the dominant kernel from the rocprofv3 --pmc summaries in
+    profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by its reading for a copy of known
+    size), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of known size (for scaling the FETCH_SIZE/WRITE_SIZE counters)."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b = a.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "sum": float(b[:8].sum())}))
+
+
 def cpu_reference_baseline(seconds=20.0):
     """The reference's own FP64 CPU path (oracle/_ref/ref_driver, all-particle DMC, nompi)
     on the host cores: one single-threaded process per core on a bounded sample."""
@@ -176,9 +216,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "fp64", "allparticle"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy three times (counter calibration) and exit")
     ap.add_argument("--quick", action="store_true",
                     help="only the named workload (no FP64 leg, no E_L comparison, no CPU)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (walkers shard "
               "trivially across GPUs; see DESIGN.md)", file=sys.stderr)
@@ -196,14 +241,17 @@ def main():
         extra["fp64_gpu_value"] = fp["value"]
         extra["mixed_over_fp64"] = main_run["value"] / fp["value"]
         extra["elocal_rel_err"] = elocal_error()
-    f = flops_per_walker_step(a.workload)
+    sm = a.workload != "allparticle"
+    f = flops_per_walker_step(a.workload, part="moves" if sm else "step")
     achieved = f * main_run["pop_mean"] / (main_run["kernel_ms"] * 1e-3) / 1e12
+    kname = {"mixed": "k_sm_moves<33,float>", "fp64": "k_sm_moves<33,double>",
+             "allparticle": "k_step<33>"}[a.workload]
     roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                kernel="k_sm_step" if a.workload != "allparticle" else "k_step",
-                kernel_ms=main_run["kernel_ms"], flops_per_walker_step=f,
-                note="FP64 vector-ALU peak (no MFMA is used; 78.6 TF/s is MI355X's FP64 "
-                     "peak for both)")
+                frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic_from_profiles(a.workload),
+                kernel=kname, kernel_ms=main_run["kernel_ms"], flops_per_launch=f *
+                main_run["pop_mean"],
+                note="vector-ALU peak: no MFMA is used; 78.6 TF/s is MI355X's FP64 peak and "
+                     "the non-packed FP32 VALU rate (packed FP32 is not emitted)")
     cpu = None
     if not a.no_cpu:
         cpu = cpu_reference_baseline()

@@ -68,6 +68,8 @@ struct dmc_ctx {
     double time_ms = 0.0;
     int64_t launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int kEv = 16;  // event pairs around the move-kernel chunks of one step
+    hipEvent_t evs[2 * kEv] = {};
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int64_t acc_moves = 0, prop_moves = 0;
@@ -291,7 +293,9 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     c->ds.jc1 = jc1;
     c->ds.ja2 = ja2;
     c->ds.jc2 = jc2;
-    if (hipHostMalloc(reinterpret_cast<void **>(&c->h_nt), 4 * sizeof(int)) != hipSuccess ||
+    bool evok = true;
+    for (hipEvent_t &e : c->evs) evok = evok && hipEventCreate(&e) == hipSuccess;
+    if (!evok || hipHostMalloc(reinterpret_cast<void **>(&c->h_nt), 4 * sizeof(int)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         dmc_destroy(c);
         return fail(DMC_ERR_HIP, "host/event allocation failed");
@@ -328,6 +332,8 @@ int dmc_destroy(dmc_ctx *c) {
     if (c->h_nt) (void)hipHostFree(c->h_nt);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t &e : c->evs)
+        if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return DMC_OK;
@@ -488,7 +494,9 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
     if (n <= 0) return fail(DMC_ERR_ARG, "no walkers loaded");
     int e = set_count(c, n);
     if (e) return e;
-    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
+    // the dominant kernel is timed with events on the stream it runs on: the move kernel
+    // (every chunk) for single-particle moves, the fused step kernel otherwise
+    int nev = 0;
     if (c->sys.p.movetype == DMC_SINGLEPARTICLE) {
         // moves in chunks of krefresh, an FP64 refresh of the inverse between chunks, then
         // the end-of-sweep FP64 evaluation; A.sinv serves as the working inverse
@@ -497,22 +505,27 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
         for (int m0 = 0; m0 < np; m0 += k) {
             const int m1 = m0 + k < np ? m0 + k : np;
             if (m0 > 0) LAUNCHCHK(launch_sm_refresh(c->sys.n, c->ds, c->B.x, c->A.sinv, c->T, n, c->st));
+            const bool ev = c->timing && nev < dmc_ctx::kEv;
+            if (ev) HIPCHK(hipEventRecord(c->evs[2 * nev], c->st));
             LAUNCHCHK(launch_sm_moves(c->sys.n, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
                                       m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1,
                                       c->T, n, c->st));
+            if (ev) HIPCHK(hipEventRecord(c->evs[2 * nev++ + 1], c->st));
         }
         LAUNCHCHK(launch_sm_end(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
     } else {
+        if (c->timing) HIPCHK(hipEventRecord(c->evs[0], c->st));
         LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
+        if (c->timing) HIPCHK(hipEventRecord(c->evs[1], c->st));
+        nev = c->timing ? 1 : 0;
     }
-    if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->st));
     LAUNCHCHK(launch_est_reduce(c->T, n, c->est_acc, c->st));
     int total = 0;
     e = compact(c, c->B, c->A, n, total);
     if (e) return e;
-    if (c->timing) {
+    for (int i = 0; i < nev; ++i) {  // the stream was synchronised by compact()
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * i], c->evs[2 * i + 1]));
         c->time_ms += ms;
         c->launches += 1;
     }

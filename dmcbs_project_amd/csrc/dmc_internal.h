@@ -32,6 +32,8 @@ struct System {
     std::vector<int> kshell;                // shell of each vector
     // tables [0..ntab]
     std::vector<double> vt, ut, dut, d2ut, bt, dbt, d2bt;
+    // d/db and d/dd of the beta tables (parameter adjoint)
+    std::vector<double> bt_b, dbt_b, d2bt_b, bt_d, dbt_d, d2bt_d;
     double b = 0, d = 0;                    // beta parameters
     std::vector<double> vorb;
 };

@@ -78,17 +78,49 @@ int kshells(System &s, std::string &err) {
     return DMC_OK;
 }
 
-// beta0 and beta (betafun.f90:412-456) for the current (b, d, c)
+// value with its derivatives with respect to (b, d): forward-mode differentiation of the
+// beta table (its d/db and d/dd tables feed the parameter adjoint)
+struct Dual {
+    double v = 0, b = 0, d = 0;
+    Dual() = default;
+    Dual(double x) : v(x) {}  // NOLINT: constants promote
+    Dual(double x, double xb, double xd) : v(x), b(xb), d(xd) {}
+};
+inline Dual operator+(Dual x, Dual y) { return {x.v + y.v, x.b + y.b, x.d + y.d}; }
+inline Dual operator-(Dual x, Dual y) { return {x.v - y.v, x.b - y.b, x.d - y.d}; }
+inline Dual operator-(Dual x) { return {-x.v, -x.b, -x.d}; }
+inline Dual operator*(Dual x, Dual y) {
+    return {x.v * y.v, x.b * y.v + x.v * y.b, x.d * y.v + x.v * y.d};
+}
+inline Dual operator/(Dual x, Dual y) {
+    const double q = x.v / y.v;
+    return {q, (x.b - q * y.b) / y.v, (x.d - q * y.d) / y.v};
+}
+inline Dual &operator+=(Dual &x, Dual y) { return x = x + y; }
+inline Dual &operator-=(Dual &x, Dual y) { return x = x - y; }
+inline Dual &operator*=(Dual &x, Dual y) { return x = x * y; }
+inline Dual &operator/=(Dual &x, Dual y) { return x = x / y; }
+inline Dual exp(Dual x) {
+    const double e = std::exp(x.v);
+    return {e, e * x.b, e * x.d};
+}
+inline double val(double x) { return x; }
+inline double val(const Dual &x) { return x.v; }
+
+// beta0 and beta (betafun.f90:412-456) for the current (b, d, c); T = double or Dual
+template <class T>
 struct Beta {
-    double b, d, c, c0, c1, c2, c3, c4, small, el;
-    void at0(double r, double &f, double &fp, double &fpp) const {
-        const double x = b * r;
+    T b, d, c, c0, c1, c2, c3, c4;
+    double small, el;
+    void at0(double r, T &f, T &fp, T &fpp) const {
+        using std::exp;
+        const T x = b * r;
         if (r < small) {
             f = c0 + x * (c1 + x * (c2 + x * (c3 + x * c4)));
             fp = c1 + x * (2.0 * c2 + x * (3.0 * c3 + 4.0 * c4 * x));
             fpp = 2.0 * c2 + x * (6.0 * c3 + 12.0 * c4 * x);
         } else {
-            const double ex = std::exp(-x), ed = std::exp(-d * x);
+            const T ex = exp(-x), ed = exp(-(d * x));
             f = (1.0 + c * x) * (1.0 - ed) * ex / x;
             fp = ex * (-1.0 - x * (1.0 + c * x) + ed * (1.0 + x * (1.0 + d) * (1.0 + c * x))) / (x * x);
             fpp = ex * (2.0 + x * (2.0 + x * (1.0 + c * x)) -
@@ -101,9 +133,9 @@ struct Beta {
         fp /= d;
         fpp /= d;
     }
-    void at(double r, double &f, double &fp, double &fpp) const {
+    void at(double r, T &f, T &fp, T &fpp) const {
         if (r > 0.5 * el) { f = fp = fpp = 0.0; return; }
-        double h0, t1, t2, f1, f1p, f1pp, f2, f2p, f2pp;
+        T h0, t1, t2, f1, f1p, f1pp, f2, f2p, f2pp;
         at0(0.5 * el, h0, t1, t2);
         at0(r, f1, f1p, f1pp);
         at0(el - r, f2, f2p, f2pp);
@@ -214,63 +246,95 @@ int jastrow_tables(System &s, std::string &err) {
 
 }  // namespace
 
-void build_beta(System &s, double b, double d) {
+namespace {
+
+// beta table (initbeta, betafun.f90:359-410) for parameters (b, d) of type T
+template <class T>
+void beta_table(const System &s, T b, T d, std::vector<T> &bt, std::vector<T> &dbt,
+                std::vector<T> &d2bt) {
+    using std::exp;
     const double el = s.el;
     const int nt = s.ntab;
-    s.b = b;
-    s.d = d;
-    Beta B{};
+    Beta<T> B{};
     B.b = b;
     B.d = d;
     B.el = el;
-    const double e1 = std::exp(b * el * (1.0 + d)), e2 = std::exp(d * b * el);
-    B.c = 0.5 * (2.0 + 2.0 * d * b * el + (d * b * el) * (d * b * el) * e1 + 2.0 * d * b * b * el * el * e1 +
-                 2.0 * b * el - 2.0 * e2 * b * el - 2.0 * e2) /
+    const T e1 = exp(b * el * (1.0 + d)), e2 = exp(d * b * el);
+    B.c = 0.5 * (2.0 + 2.0 * d * b * el + (d * b * el) * (d * b * el) * e1 +
+                 2.0 * d * b * b * el * el * e1 + 2.0 * b * el - 2.0 * e2 * b * el - 2.0 * e2) /
           (el * el * (e2 - 1.0 - d + d * e1) * b * b);
-    const double c = B.c;
+    const T c = B.c;
     B.c0 = d;
-    B.small = 1.0e-3 / b;
+    B.small = 1.0e-3 / val(b);
     B.c1 = -d + c * d - d * d / 2.0;
     B.c2 = d / 2.0 + d * d / 2.0 + d * d * d / 6.0 + c * (-d - d * d / 2.0);
-    B.c3 = -d / 6.0 - d * d / 4.0 - d * d * d / 6.0 - d * d * d * d / 24.0 + c * (d / 2.0 + d * d / 2.0 + d * d * d / 6.0);
-    B.c4 = d / 24.0 + d * d / 12.0 + d * d * d / 12.0 + d * d * d * d / 24.0 + d * d * d * d * d / 120.0 -
-           c * (d / 6.0 + d * d / 4.0 + d * d * d / 6.0 + d * d * d * d / 24.0);
+    B.c3 = -d / 6.0 - d * d / 4.0 - d * d * d / 6.0 - d * d * d * d / 24.0 +
+           c * (d / 2.0 + d * d / 2.0 + d * d * d / 6.0);
+    B.c4 = d / 24.0 + d * d / 12.0 + d * d * d / 12.0 + d * d * d * d / 24.0 +
+           d * d * d * d * d / 120.0 - c * (d / 6.0 + d * d / 4.0 + d * d * d / 6.0 + d * d * d * d / 24.0);
     // number of periodic images needed (betafun.f90:379-384)
     int nj = 0;
     for (double bn = 10.0; std::fabs(bn) > 1.0e-10;) {
         ++nj;
-        double t1, t2;
-        B.at(el * nj, bn, t1, t2);
+        T v, t1, t2;
+        B.at(el * nj, v, t1, t2);
+        bn = val(v);
     }
-    s.bt.assign(nt + 1, 0.0);
-    s.dbt.assign(nt + 1, 0.0);
-    s.d2bt.assign(nt + 1, 0.0);
+    bt.assign(nt + 1, T(0.0));
+    dbt.assign(nt + 1, T(0.0));
+    d2bt.assign(nt + 1, T(0.0));
     const double dr = 0.5 * el / nt;
     for (int i = 0; i <= nt; ++i) {
         const double r = i * dr;
-        double f, fp, fpp;
+        T f, fp, fpp;
         B.at(r, f, fp, fpp);
         for (int j = 1; j <= nj; ++j) {
-            double a0, a1, a2, c0_, c1_, c2_;
+            T a0, a1, a2, c0_, c1_, c2_;
             B.at(j * el - r, a0, a1, a2);
             B.at(j * el + r, c0_, c1_, c2_);
             f += a0 + c0_;
             fp += -a1 + c1_;
             fpp += a2 + c2_;
         }
-        s.bt[i] = f;
-        s.dbt[i] = fp;
-        s.d2bt[i] = fpp;
+        bt[i] = f;
+        dbt[i] = fp;
+        d2bt[i] = fpp;
     }
-    const double top = s.bt[nt];
+    const T top = bt[nt];
     const double fac = std::fabs(s.p.bfact);
     for (int i = 0; i <= nt; ++i) {
-        s.bt[i] -= top;
+        bt[i] -= top;
         // store the radial laplacian f'' + 2 f'/r (3 f'' at the origin)
-        s.d2bt[i] = i ? s.d2bt[i] + 2.0 * s.dbt[i] / (i * dr) : 3.0 * s.d2bt[i];
-        s.bt[i] *= fac;
-        s.dbt[i] *= fac;
-        s.d2bt[i] *= fac;
+        d2bt[i] = i ? d2bt[i] + 2.0 * dbt[i] / (i * dr) : 3.0 * d2bt[i];
+        bt[i] *= fac;
+        dbt[i] *= fac;
+        d2bt[i] *= fac;
+    }
+}
+
+}  // namespace
+
+void build_beta(System &s, double b, double d) {
+    s.b = b;
+    s.d = d;
+    beta_table<double>(s, b, d, s.bt, s.dbt, s.d2bt);
+    // derivative tables with respect to b and d
+    std::vector<Dual> t0, t1, t2;
+    beta_table<Dual>(s, Dual(b, 1.0, 0.0), Dual(d, 0.0, 1.0), t0, t1, t2);
+    const int n1 = s.ntab + 1;
+    s.bt_b.resize(n1);
+    s.dbt_b.resize(n1);
+    s.d2bt_b.resize(n1);
+    s.bt_d.resize(n1);
+    s.dbt_d.resize(n1);
+    s.d2bt_d.resize(n1);
+    for (int i = 0; i < n1; ++i) {
+        s.bt_b[i] = t0[i].b;
+        s.dbt_b[i] = t1[i].b;
+        s.d2bt_b[i] = t2[i].b;
+        s.bt_d[i] = t0[i].d;
+        s.dbt_d[i] = t1[i].d;
+        s.d2bt_d[i] = t2[i].d;
     }
 }
 
@@ -321,8 +385,12 @@ int build_system(const dmc_params &p, System &s, std::string &err) {
         err = "beta table from file (bfact < 0) is not supported; use dmc_set_tables";
         return DMC_ERR_ARG;
     }
-    if (!s.nobeta) build_beta(s, p.bbeta, p.dbeta);
-    else { s.bt.assign(p.ntab + 1, 0.0); s.dbt = s.bt; s.d2bt = s.bt; }
+    if (!s.nobeta) {
+        build_beta(s, p.bbeta, p.dbeta);
+    } else {
+        s.bt.assign(p.ntab + 1, 0.0);
+        s.dbt = s.d2bt = s.bt_b = s.dbt_b = s.d2bt_b = s.bt_d = s.dbt_d = s.d2bt_d = s.bt;
+    }
     return jastrow_tables(s, err);
 }
 

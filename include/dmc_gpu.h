@@ -117,9 +117,10 @@ int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed);
  * recomputed, not stored.  dlogpsi and delocal are [nw][norb+2]; delocal may be NULL. */
 int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double *delocal);
 
-/* Timing of the dominant kernel (the fused step kernel) with HIP events recorded on the
- * stream it is launched on: enable = 1 starts collecting, 0 stops.  Returns the summed
- * milliseconds and the number of launches since the last reset. */
+/* Timing of the dominant kernel with HIP events recorded on the stream it is launched on:
+ * the move kernel (k_sm_moves, every chunk of krefresh moves) for single-particle moves, the
+ * fused step kernel (k_step) for all-particle moves.  enable = 1 starts collecting, 0 stops.
+ * Returns the summed milliseconds and the number of launches since the last reset. */
 int dmc_kernel_timing(dmc_ctx *c, int32_t enable);
 int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset);
 /* Synchronise the context's stream. */
