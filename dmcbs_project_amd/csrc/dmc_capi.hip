@@ -108,14 +108,20 @@ namespace {
 int upload_tables(dmc_ctx *c) {
     const System &s = c->sys;
     const int n1 = s.ntab + 1;
-    std::vector<double4> jt(n1), bt(n1);
+    std::vector<double4> jt(n1), bt(n1), btb(n1), btd(n1);
     std::vector<float4> jt32(n1), bt32(n1);
     for (int i = 0; i < n1; ++i) {
         jt[i] = make_double4(s.vt[i], s.ut[i], s.dut[i], s.d2ut[i]);
         bt[i] = make_double4(s.bt[i], s.dbt[i], s.d2bt[i], 0.0);
+        btb[i] = make_double4(s.bt_b[i], s.dbt_b[i], s.d2bt_b[i], 0.0);
+        btd[i] = make_double4(s.bt_d[i], s.dbt_d[i], s.d2bt_d[i], 0.0);
         jt32[i] = make_float4(float(s.vt[i]), float(s.ut[i]), float(s.dut[i]), float(s.d2ut[i]));
         bt32[i] = make_float4(float(s.bt[i]), float(s.dbt[i]), float(s.d2bt[i]), 0.0f);
     }
+    HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.btb), btb.data(), sizeof(double4) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.btd), btd.data(), sizeof(double4) * n1,
+                          hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.jt), jt.data(), sizeof(double4) * n1,
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.bt), bt.data(), sizeof(double4) * n1,
@@ -265,12 +271,13 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     }
     fill_devsys(c);
     const int n1 = s.ntab + 1, nq = s.nk - 1;
-    double4 *jt, *bt;
+    double4 *jt, *bt, *btb, *btd;
     float4 *jt32, *bt32;
     int4 *qm;
     double *wq;
     uint64_t *ja1, *jc1, *ja2, *jc2;
-    if (c->alloc(&jt, n1) || c->alloc(&bt, n1) || c->alloc(&jt32, n1) || c->alloc(&bt32, n1) ||
+    if (c->alloc(&jt, n1) || c->alloc(&bt, n1) || c->alloc(&btb, n1) || c->alloc(&btd, n1) ||
+        c->alloc(&jt32, n1) || c->alloc(&bt32, n1) ||
         c->alloc(&qm, nq) || c->alloc(&wq, nq) || c->alloc(&ja1, kMaxDraw) ||
         c->alloc(&jc1, kMaxDraw) || c->alloc(&ja2, kMaxDraw) || c->alloc(&jc2, kMaxDraw) ||
         c->alloc_state(c->A) || c->alloc_state(c->B) || c->alloc(&c->T.count, c->capacity) ||
@@ -285,6 +292,8 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     }
     c->ds.jt = jt;
     c->ds.bt = bt;
+    c->ds.btb = btb;
+    c->ds.btd = btd;
     c->ds.jt32 = jt32;
     c->ds.bt32 = bt32;
     c->ds.qm = qm;
@@ -366,6 +375,9 @@ int dmc_set_tables(dmc_ctx *c, const double *vt, const double *ut, const double 
     s.bt.assign(bt, bt + n1);
     s.dbt.assign(dbt, dbt + n1);
     s.d2bt.assign(d2bt, d2bt + n1);
+    // a table from the host carries no (b, d) dependence: those derivatives become NaN
+    const double nan = std::nan("");
+    for (auto *t : {&s.bt_b, &s.dbt_b, &s.d2bt_b, &s.bt_d, &s.dbt_d, &s.d2bt_d}) t->assign(n1, nan);
     return upload_tables(c);
 }
 
@@ -607,9 +619,22 @@ int dmc_sync(dmc_ctx *c) {
     return DMC_OK;
 }
 
-int dmc_adjoint(dmc_ctx *c, int32_t, const double *, double *, double *) {
-    if (!c) return fail(DMC_ERR_ARG, "null context");
-    return fail(DMC_ERR_ARG, "adjoint not available");
+int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double *delocal) {
+    if (!c || !x || !dlogpsi || nw < 0) return fail(DMC_ERR_ARG, "invalid argument");
+    if (nw == 0) return DMC_OK;
+    const size_t m = size_t(3) * c->sys.npart, np = size_t(c->sys.p.norb) + 2;
+    const size_t bytes = sizeof(double) * (m * nw + 2 * np * nw) + 256;
+    int e = c->ensure_scratch(bytes);
+    if (e) return e;
+    double *dx = static_cast<double *>(c->scratch), *dl = dx + m * nw, *de = dl + np * nw;
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * m * nw, hipMemcpyHostToDevice, c->st));
+    LAUNCHCHK(launch_adjoint(c->sys.n, c->ds, dx, nw, c->sys.p.norb, dl, delocal ? de : nullptr,
+                             c->st));
+    HIPCHK(hipMemcpyAsync(dlogpsi, dl, sizeof(double) * np * nw, hipMemcpyDeviceToHost, c->st));
+    if (delocal)
+        HIPCHK(hipMemcpyAsync(delocal, de, sizeof(double) * np * nw, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
 }
 
 }  // extern "C"

@@ -18,6 +18,8 @@ struct DevSys {
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
     const float4 *jt32;   // FP32 copies for the mixed-precision moves
     const float4 *bt32;
+    const double4 *btb;   // d/db and d/dd of the beta records (parameter adjoint)
+    const double4 *btd;
     const int4 *qm;       // integer q vectors (mx, my, mz, shell), [nq]
     const double *wq;     // 2 alpha_shell(q), [nq]
     const uint64_t *ja1;  // ran1 jump-ahead: state after t+1 draws = ja1[t]*s + jc1[t]

@@ -9,7 +9,7 @@ constexpr uint64_t kM1 = 44485709377909ULL, kA1 = 96309754297ULL;  // ran1 (ran.
 constexpr uint64_t kM2 = 34522712143931ULL, kA2 = 55789347517ULL;  // ran2 (ran.f90:36,39)
 constexpr double kPi = 3.14159265358979323846;
 
-// Host launchers (dmc_launch.hip).  Each returns a hipError_t value.
+// Host launchers.  Each returns a hipError_t value.
 int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
                 double *dpsi, double *d2psi, double *v, char *sinv, hipStream_t st);
 int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const double *xnew,
@@ -27,8 +27,9 @@ int launch_sm_refresh(int nup, const DevSys &S, const double *x, char *sinv, con
 // end of the sweep: FP64 hpsi of B.x, inverse into B.sinv, weights against A, branch draw
 int launch_sm_end(int nup, const DevSys &S, const DevState &A, const DevState &B,
                   const DevStep &T, int opcomp, int grid, hipStream_t st);
-int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, const double *dbeta,
-                   double *dlogpsi, double *delocal, hipStream_t st);
+// parameter adjoint of nw configurations: d log psi and d E_L per parameter, [nw][norb + 2]
+int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, int norb, double *dlogpsi,
+                   double *delocal, hipStream_t st);
 // branching: multiplicities of state A with factor (population control), prefix sums,
 // the destination map and the gather into state B
 int launch_branch_count(const DevSys &S, const DevState &A, const DevStep &T, double factor,

@@ -528,9 +528,4 @@ int launch_sm_end(int nup, const DevSys &S, const DevState &A, const DevState &B
     return hipErrorInvalidValue;
 }
 
-int launch_adjoint(int, const DevSys &, const double *, int, const double *, double *, double *,
-                   hipStream_t) {
-    return hipErrorNotSupported;
-}
-
 }  // namespace dmc

@@ -57,7 +57,7 @@ def lib():
         L.dmco_hpsi.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.dmco_ratio1.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
         L.dmco_ratio1.restype = C.c_double
-        L.dmco_getder.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.dmco_getder.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
         L.dmco_set_params.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double]
         L.dmco_ran1.argtypes = [C.POINTER(C.c_int64)]
         L.dmco_ran1.restype = C.c_double
@@ -183,10 +183,11 @@ class System:
         x = np.ascontiguousarray(x, np.float64).reshape(-1)
         return self.L.dmco_ratio1(self.h, x, int(ip), np.ascontiguousarray(xnew, np.float64))
 
-    def getder(self, x):
+    def getder(self, x, optbeta=False):
+        """d log psi / d (alpha_1..alpha_norb[, b, d]) (getderbcs)."""
         x = np.ascontiguousarray(x, np.float64).reshape(-1)
-        o = np.zeros(self.params["norb"])
-        self.L.dmco_getder(self.h, x, o)
+        o = np.zeros(self.params["norb"] + (2 if optbeta else 0))
+        self.L.dmco_getder(self.h, x, int(bool(optbeta)), o)
         return o
 
     def set_params(self, vorb, bbeta, dbeta):

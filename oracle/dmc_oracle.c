@@ -746,13 +746,16 @@ double dmco_ratio1(const dmco_ctx *c, const double *x, int32_t ip, const double 
     return r;
 }
 
-/* psislaterbcs.f90:449-524 getderbcs with optsw, no optbeta */
-void dmco_getder(const dmco_ctx *c, const double *x, double *dlogpsi) {
-    int n = c->n, norb = c->p.norb;
+/* psislaterbcs.f90:449-524 getderbcs with optsw and, when optbeta is set, optbeta:
+ * d log psi / d (alpha_1 .. alpha_norb [, b, d]) = sum_ij Sinv(j, i) dS(i, j), with
+ * dS = 1 for alpha_1, 2 sum_{k in shell} cos(k.r_ij) for the other shells and the closed-form
+ * derbeta (betafun.f90:196-327) for b and d (psislaterbcs.f90:510-517). */
+void dmco_getder(const dmco_ctx *c, const double *x, int32_t optbeta, double *dlogpsi) {
+    int n = c->n, norb = c->p.norb, np = norb + (optbeta ? 2 : 0);
     double *s = malloc(sizeof(double) * n * n), phil;
     int32_t is = 1;
     allpair(c, x, s, NULL, NULL);
-    for (int p = 0; p < norb; p++) dlogpsi[p] = 0.0;
+    for (int p = 0; p < np; p++) dlogpsi[p] = 0.0;
     if (matinv(s, n, &phil, &is)) { free(s); return; }
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) {
@@ -766,6 +769,12 @@ void dmco_getder(const dmco_ctx *c, const double *x, double *dlogpsi) {
                 for (int kk = 0; kk < c->numsh[ksh]; kk++, k++)
                     acc += 2.0 * cos(c->ak[k][0] * dx[0] + c->ak[k][1] * dx[1] + c->ak[k][2] * dx[2]);
                 dlogpsi[ksh] += sij * acc;
+            }
+            if (optbeta) {
+                double db[2];
+                dmco_derbeta(c, sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]), db);
+                dlogpsi[norb] += sij * db[0];
+                dlogpsi[norb + 1] += sij * db[1];
             }
         }
     free(s);

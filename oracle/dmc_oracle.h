@@ -53,7 +53,7 @@ void dmco_hpsi(const dmco_ctx *c, const double *x, double *out, double *dpsi);
 /* psi(x with particle ip moved to xnew)/psi(x), ip 0-based (wavefunction.f90:393-404) */
 double dmco_ratio1(const dmco_ctx *c, const double *x, int32_t ip, const double *xnew);
 /* d log psi / d alpha_s, s = 1..norb (psislaterbcs.f90:449-524, optsw, no optbeta) */
-void dmco_getder(const dmco_ctx *c, const double *x, double *dlogpsi);
+void dmco_getder(const dmco_ctx *c, const double *x, int32_t optbeta, double *dlogpsi);
 /* set alpha_1..alpha_norb and (b, d) of beta; rebuilds the beta table (setbcspar) */
 void dmco_set_params(dmco_ctx *c, const double *vorb, double bbeta, double dbeta);
 
