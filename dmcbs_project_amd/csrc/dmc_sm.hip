@@ -38,34 +38,35 @@ struct Vec2<float> {
     using T = float2;
 };
 
-// lane exchanges inside a 16-lane row by DPP (xor 1, xor 2, mirror within 8, mirror within
-// 16), then across rows by shuffles: the sum of all 64 lanes in every lane
-__device__ __forceinline__ int dpp_i(int v, int ctrl) {
-    switch (ctrl) {
-        case 0: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-        case 1: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
-        case 2: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-        default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror
-    }
+// minimum image with round-to-nearest-even (a tie has measure zero for these positions)
+__device__ __forceinline__ double wrapr(double x, const DevSys &S) {
+    return x - S.el * rint(x * S.eli);
 }
-__device__ __forceinline__ float wsum(float v) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v += __int_as_float(dpp_i(__float_as_int(v), k));
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+
+// square root and reciprocal: correctly rounded in FP64, the hardware approximations
+// (about 1 ulp) in FP32
+template <typename R>
+__device__ __forceinline__ R sqrt_r(R x) {
+    if constexpr (sizeof(R) == 8) return sqrt(x);
+    else return __builtin_amdgcn_sqrtf(x);
 }
-__device__ __forceinline__ double wsum(double v) {
+template <typename R>
+__device__ __forceinline__ R rcp_r(R x) {
+    if constexpr (sizeof(R) == 8) return 1.0 / x;
+    else return __builtin_amdgcn_rcpf(x);
+}
+
+// distance of the pair (a, b), the minimum-image difference formed in FP64 and the rest in R
+template <typename R>
+__device__ __forceinline__ R pair_dist(const DevSys &S, const double *xa, const double *xb,
+                                       R (&dx)[3]) {
+    R r2 = R(0);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const long long b = __double_as_longlong(v);
-        const int lo = dpp_i(int(b & 0xffffffffll), k), hi = dpp_i(int(b >> 32), k);
-        v += __longlong_as_double((static_cast<long long>(hi) << 32) |
-                                  static_cast<unsigned int>(lo));
+    for (int d = 0; d < 3; ++d) {
+        dx[d] = R(wrapr(xa[d] - xb[d], S));
+        r2 += dx[d] * dx[d];
     }
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+    return sqrt_r<R>(r2);
 }
 
 template <typename R, typename V>
@@ -236,10 +237,11 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
     }
     __syncthreads();
 
-    const double sig = S.sigma, sig2 = S.sig2;
+    const double sig = S.sigma, sig2 = S.sig2, half_isig2 = 0.5 / S.sig2;
     const R alpha0 = R(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
+    const int r0 = tid / NUP, c0 = tid - (tid / NUP) * NUP;
     for (int ip = m0; ip < m1; ++ip) {
         const bool up = ip < NUP;
         const int a = up ? ip : ip - NUP;
@@ -251,13 +253,8 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
             const int b = tid;
             pb = up ? NUP + b : b;
             wgt = up ? sv[b * SP + a] : sv[a * SP + b];
-            double dx[3], r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                dx[d] = wrapd(xa[d] - xs[3 * pb + d], S);
-                r2 += dx[d] * dx[d];
-            }
-            const R ro = R(sqrt(r2));
+            R dx[3];
+            const R ro = pair_dist<R>(S, xa, xs + 3 * pb, dx);
             // table values first, so that their loads overlap the plane-wave sum
             R bv = 0, db = 0, uu = 0, du = 0;
             const R rj = ro > R(1.0e-14) ? ro : R(1.0e-14);
@@ -271,14 +268,14 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 go1 += qc[5 * q + 2] * sab;
                 go2 += qc[5 * q + 3] * sab;
             }
-            const R t = db / ro, tj = du / rj;
+            const R t = db * rcp_r<R>(ro), tj = du * rcp_r<R>(rj);
             uo = uu;
             // determinant part weighted by the inverse, plus the Jastrow part
-            go0 = wgt * (go0 + t * R(dx[0])) + R(dx[0]) * tj;
-            go1 = wgt * (go1 + t * R(dx[1])) + R(dx[1]) * tj;
-            go2 = wgt * (go2 + t * R(dx[2])) + R(dx[2]) * tj;
+            go0 = wgt * (go0 + t * dx[0]) + dx[0] * tj;
+            go1 = wgt * (go1 + t * dx[1]) + dx[1] * tj;
+            go2 = wgt * (go2 + t * dx[2]) + dx[2] * tj;
         }
-        const double go[3] = {double(wsum(go0)), double(wsum(go1)), double(wsum(go2))};
+        const double go[3] = {double(wave_sum(go0)), double(wave_sum(go1)), double(wave_sum(go2))};
         // gaussian(3) then the acceptance draw: five ran1 draws per move
         double u[5];
 #pragma unroll
@@ -307,7 +304,7 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
             dro[d] = go[d] * sig2;
             if (S.cutdrift) dro[d] = fmin(S.dcut, fmax(-S.dcut, dro[d]));
             dxm[d] = dro[d] + sig * g3[d];
-            xn[d] = wrapd(xa[d] + dxm[d], S);
+            xn[d] = wrapr(xa[d] + dxm[d], S);
         }
         // phases of the proposed position: powers per axis (lanes 0..2), then lanes over q
         if (tid < 3) {
@@ -326,13 +323,8 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
         double rho_part = 0.0;
         R gn0 = 0, gn1 = 0, gn2 = 0, jn0 = 0, jn1 = 0, jn2 = 0, dU = 0;
         if (tid < NUP) {
-            double dx[3], r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                dx[d] = wrapd(xn[d] - xs[3 * pb + d], S);
-                r2 += dx[d] * dx[d];
-            }
-            const R rn = R(sqrt(r2));
+            R dx[3];
+            const R rn = pair_dist<R>(S, xn, xs + 3 * pb, dx);
             R bv = 0, db = 0, uu = 0, du = 0;
             const R rj = rn > R(1.0e-14) ? rn : R(1.0e-14);
             if (!S.nobeta && rn <= R(S.rangeb)) tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db);
@@ -349,39 +341,40 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
                 g2 += qc[5 * q + 3] * sab;
             }
             f += alpha0 + bv;
-            const R t = db / rn;
-            g0 += t * R(dx[0]);
-            g1 += t * R(dx[1]);
-            g2 += t * R(dx[2]);
+            const R t = db * rcp_r<R>(rn);
+            g0 += t * dx[0];
+            g1 += t * dx[1];
+            g2 += t * dx[2];
             phinew[tid] = f;
             rho_part = double(wgt) * double(f);
             gn0 = wgt * g0;
             gn1 = wgt * g1;
             gn2 = wgt * g2;
-            const R tj = du / rj;
-            jn0 = R(dx[0]) * tj;
-            jn1 = R(dx[1]) * tj;
-            jn2 = R(dx[2]) * tj;
+            const R tj = du * rcp_r<R>(rj);
+            jn0 = dx[0] * tj;
+            jn1 = dx[1] * tj;
+            jn2 = dx[2] * tj;
             dU = uu - uo;
         }
         const double rho = wave_sum(rho_part);  // FP64 sum: the node test uses its sign
-        const double gn[3] = {double(wsum(gn0)), double(wsum(gn1)), double(wsum(gn2))};
-        const double jn[3] = {double(wsum(jn0)), double(wsum(jn1)), double(wsum(jn2))};
-        const double dlogj = double(wsum(dU));
+        const double gn[3] = {double(wave_sum(gn0)), double(wave_sum(gn1)), double(wave_sum(gn2))};
+        const double jn[3] = {double(wave_sum(jn0)), double(wave_sum(jn1)), double(wave_sum(jn2))};
+        const double dlogj = double(wave_sum(dU));
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
             double logp = 2.0 * (log(fabs(rho)) + dlogj);
+            const double irho = 1.0 / rho;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                double drn = (gn[d] / rho + jn[d]) * sig2;
+                double drn = (gn[d] * irho + jn[d]) * sig2;
                 if (S.cutdrift) drn = fmin(S.dcut, fmax(-S.dcut, drn));
-                logp += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]) / (2.0 * sig2);
+                logp += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]) * half_isig2;
             }
             accept = u[4] < exp(fmin(0.0, logp));
         }
         if (accept) {
             ++nacc;
-            const R rr = R(rho);
+            const R irr = R(1.0 / rho);
             // capture column a (up mover) or row a (down mover) of the inverse
             if (tid < NUP) colbuf[tid] = up ? sv[tid * SP + a] : sv[a * SP + tid];
             __syncthreads();
@@ -395,15 +388,24 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
 #pragma unroll
                     for (int b = 0; b < NUP; ++b) z += sv[k * SP + b] * phinew[b];
                 }
-                fbuf[k] = (z - (k == a ? R(1) : R(0))) / rr;
+                fbuf[k] = (z - (k == a ? R(1) : R(0))) * irr;
             }
             __syncthreads();
+            // rank-1 update over the n x n elements, 64 at a time; (r, c) of element
+            // e0 + tid advanced by 64 = kq n + kr without a division
+            {
+                constexpr int kq = 64 / NUP, kr = 64 % NUP;
+                int r = r0, c = c0;
 #pragma unroll
-            for (int e0 = 0; e0 < NUP * NUP; e0 += 64) {
-                const int e = e0 + tid;
-                if (e < NUP * NUP) {
-                    const int r = e / NUP, c = e - r * NUP;
-                    sv[r * SP + c] -= up ? colbuf[r] * fbuf[c] : fbuf[r] * colbuf[c];
+                for (int e0 = 0; e0 < NUP * NUP; e0 += 64) {
+                    if (e0 + tid < NUP * NUP)
+                        sv[r * SP + c] -= up ? colbuf[r] * fbuf[c] : fbuf[r] * colbuf[c];
+                    r += kq;
+                    c += kr;
+                    if (c >= NUP) {
+                        c -= NUP;
+                        ++r;
+                    }
                 }
             }
             if (tid < 3) xs[3 * ip + tid] = xn[tid];

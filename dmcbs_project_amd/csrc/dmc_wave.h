@@ -46,10 +46,47 @@ __device__ __forceinline__ double beta_val(const DevSys &S, double r) {
     return w0 * S.bt[idx - 1].x + w1 * S.bt[idx].x + w2 * S.bt[idx + 1].x + w3 * S.bt[idx + 2].x;
 }
 
+// Sum over the 64 lanes of a wave, returned as a wave-uniform value.  Within each 16-lane
+// row by DPP lane exchanges (xor 1, xor 2, mirror within 8, mirror within 16), then rows
+// combined by row_bcast:15 (rows 1, 3 add the last lane of the row before) and row_bcast:31
+// (rows 2, 3 add lane 31), so that lane 63 holds the total, read back as a scalar.  All 64
+// lanes must be active at the call.
+__device__ __forceinline__ int dpp_row(int v, int k) {
+    switch (k) {
+        case 0: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+        default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror
+    }
+}
+__device__ __forceinline__ int dpp_bcast(int v, int k) {
+    return k == 0 ? __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false)   // row_bcast:15
+                  : __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v += __int_as_float(dpp_row(__float_as_int(v), k));
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v += __int_as_float(dpp_bcast(__float_as_int(v), k));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double dbl_of(int lo, int hi) {
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    for (int k = 0; k < 4; ++k) {
+        const long long b = __double_as_longlong(v);
+        v += dbl_of(dpp_row(int(b & 0xffffffffll), k), dpp_row(int(b >> 32), k));
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const long long b = __double_as_longlong(v);
+        v += dbl_of(dpp_bcast(int(b & 0xffffffffll), k), dpp_bcast(int(b >> 32), k));
+    }
+    const long long b = __double_as_longlong(v);
+    return dbl_of(__builtin_amdgcn_readlane(int(b & 0xffffffffll), 63),
+                  __builtin_amdgcn_readlane(int(b >> 32), 63));
 }
 
 // exp(i m theta) from the precomputed powers, |m| <= 4
