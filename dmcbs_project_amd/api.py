@@ -175,6 +175,16 @@ class Engine:
         L.check(self.lib.dmc_step(self.h, int(bool(opcomp)), nt))
         return int(nt[0]), int(nt[1])
 
+    def step1(self, opcomp=False):
+        """step1 (step.f90:43-285) on the resident population; dmc_branch must follow."""
+        L.check(self.lib.dmc_propagate(self.h, int(bool(opcomp))))
+
+    def branch(self, factor=1.0):
+        """branch (step.f90:287-315); returns the new population."""
+        nt = C.c_int32()
+        L.check(self.lib.dmc_branch(self.h, float(factor), C.byref(nt)))
+        return nt.value
+
     def run(self, nsteps):
         nt = C.c_int32()
         L.check(self.lib.dmc_run(self.h, int(nsteps), C.byref(nt)))

@@ -73,6 +73,8 @@ struct dmc_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int64_t acc_moves = 0, prop_moves = 0;
+    bool pending = false;  // a propagation whose branch (factor 1) has not run yet
+    int nev_pending = 0;   // timed move-kernel chunks of that propagation
 
     template <class T>
     int alloc(T **p, size_t n) {
@@ -405,6 +407,7 @@ int dmc_load_walkers(dmc_ctx *c, int32_t nw, const double *x, const int64_t *irn
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->nw = nw;
+    c->pending = false;
     return DMC_OK;
 }
 
@@ -425,6 +428,7 @@ int dmc_population(const dmc_ctx *c, int32_t *nw) {
 int dmc_read_walkers(dmc_ctx *c, int32_t cap, int32_t *nw, double *x, double *dpsi, double *sc,
                      int64_t *irn) {
     if (!c || !nw) return fail(DMC_ERR_ARG, "null argument");
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
     const int n = c->nw;
     *nw = n;
     if (cap < n) return fail(DMC_ERR_ARG, "output capacity smaller than the population");
@@ -500,10 +504,11 @@ int dmc_ratio1(dmc_ctx *c, int32_t nw, const double *x, const int32_t *ip, const
     return DMC_OK;
 }
 
-int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
+int dmc_propagate(dmc_ctx *c, int32_t opcomp) {
     if (!c) return fail(DMC_ERR_ARG, "null context");
     const int n = c->nw;
     if (n <= 0) return fail(DMC_ERR_ARG, "no walkers loaded");
+    if (c->pending) return fail(DMC_ERR_ARG, "dmc_propagate called twice without dmc_branch");
     int e = set_count(c, n);
     if (e) return e;
     // the dominant kernel is timed with events on the stream it runs on: the move kernel
@@ -532,27 +537,58 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
         nev = c->timing ? 1 : 0;
     }
     LAUNCHCHK(launch_est_reduce(c->T, n, c->est_acc, c->st));
-    int total = 0;
-    e = compact(c, c->B, c->A, n, total);
-    if (e) return e;
-    for (int i = 0; i < nev; ++i) {  // the stream was synchronised by compact()
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * i], c->evs[2 * i + 1]));
-        c->time_ms += ms;
-        c->launches += 1;
-    }
-    int final_n = total;
-    const double nwalk = c->sys.p.nwalk;
-    if (total > c->sys.p.popup * nwalk || total < c->sys.p.popdn * nwalk) {
-        if (total == 0) return fail(DMC_ERR_ARG, "population died out");
-        e = set_count(c, total);
+    c->pending = true;
+    c->nev_pending = nev;
+    return DMC_OK;
+}
+
+int dmc_branch(dmc_ctx *c, double factor, int32_t *nt) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    const int n = c->nw;
+    int total = 0, e = 0;
+    if (c->pending) {
+        // the multiplicities int(weight + u) were drawn by the propagation (factor 1)
+        if (factor != 1.0)
+            return fail(DMC_ERR_ARG, "the branch that follows dmc_propagate takes factor 1");
+        c->pending = false;
+        e = compact(c, c->B, c->A, n, total);
         if (e) return e;
-        LAUNCHCHK(launch_branch_count(c->ds, c->A, c->T, 1.0 * c->sys.p.nwalk / total, total, c->st));
-        e = compact(c, c->A, c->B, total, final_n);
+        for (int i = 0; i < c->nev_pending; ++i) {  // the stream was synchronised by compact()
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * i], c->evs[2 * i + 1]));
+            c->time_ms += ms;
+            c->launches += 1;
+        }
+        c->nev_pending = 0;
+    } else {
+        // population control: int(weight * factor + u) copies of every walker
+        if (n <= 0) return fail(DMC_ERR_ARG, "population died out");
+        if (!(factor > 0.0)) return fail(DMC_ERR_ARG, "branch factor must be positive");
+        e = set_count(c, n);
+        if (e) return e;
+        LAUNCHCHK(launch_branch_count(c->ds, c->A, c->T, factor, n, c->st));
+        e = compact(c, c->A, c->B, n, total);
         if (e) return e;
         std::swap(c->A, c->B);
     }
-    c->nw = final_n;
+    c->nw = total;
+    if (nt) *nt = total;
+    return DMC_OK;
+}
+
+int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
+    int e = dmc_propagate(c, opcomp);
+    if (e) return e;
+    int total = 0, final_n = 0;
+    e = dmc_branch(c, 1.0, &total);
+    if (e) return e;
+    final_n = total;
+    const double nwalk = c->sys.p.nwalk;
+    if (total > c->sys.p.popup * nwalk || total < c->sys.p.popdn * nwalk) {
+        if (total == 0) return fail(DMC_ERR_ARG, "population died out");
+        e = dmc_branch(c, nwalk / total, &final_n);
+        if (e) return e;
+    }
     if (nt) {
         nt[0] = total;
         nt[1] = final_n;

@@ -1,0 +1,388 @@
+! Fortran host modules over the MI355X engine.  They keep the module interfaces of the
+! reference's walker-propagation path so that a driver written against them reads like
+! dmcbcs.f90:
+!   potential    setphi                          (coshpot.f90:14-143)
+!   wavefunction setpsi, hpsi, getderpsi, setdetparam, getdetparam, move1
+!                                                (wavefunction.f90:13-404)
+!   stack        type(walker), create, push, pop, numstack   (stack.f90:6-119)
+!   step         setstep, step1, branch, writewtisto         (step.f90:11-334)
+!   estimator    setestnum, addest, zerest, update, resstring (estimator.f90:19-149)
+! All computation runs on the device through the C-ABI (module dmc_gpu); the walker
+! population stays in device memory.  The deck values each reference module reads from
+! unit 5 are read here in the same order and collected into one dmc_params record, and the
+! device context is created on first use after setstep.
+module engine
+   use iso_c_binding
+   use dmc_gpu
+   implicit none
+   integer, parameter :: i4=selected_int_kind(9)
+   integer, parameter :: i8=selected_int_kind(15)
+   integer, parameter :: r8=selected_real_kind(15,9)
+   type(dmc_params), save :: prm
+   type(c_ptr), save :: ctx=c_null_ptr
+   logical, save :: started=.false.
+contains
+! run-level deck values that the reference keeps in dmcbcs.f90 (raw etrial in units of the
+! Fermi energy per particle when units is set, as read at dmcbcs.f90:51-52)
+   subroutine engine_deck(nwalk,etrial,units,popup,popdn,precision,movetype,krefresh,capacity)
+   integer(kind=i4) :: nwalk,precision,movetype,krefresh,capacity
+   real(kind=r8) :: etrial,popup,popdn
+   logical :: units
+   prm%nwalk=nwalk
+   prm%etrial=etrial
+   prm%units=merge(1,0,units)
+   prm%popup=popup
+   prm%popdn=popdn
+   prm%precision=precision
+   prm%movetype=movetype
+   prm%krefresh=krefresh
+   prm%capacity=capacity
+   end subroutine engine_deck
+
+   function engine_ctx() result(c)
+   type(c_ptr) :: c
+   if (.not.started) then
+      call dmc_check(dmc_create(prm,ctx),'dmc_create')
+      started=.true.
+   endif
+   c=ctx
+   end function engine_ctx
+
+   subroutine engine_stop
+   integer(c_int) :: ierr
+   if (started) ierr=dmc_destroy(ctx)
+   started=.false.
+   ctx=c_null_ptr
+   end subroutine engine_stop
+end module engine
+
+module potential
+   use engine
+   implicit none
+contains
+! setphi: reads v0, amu, cjas(1:2), djas, v0op, v0eq (coshpot.f90:35-41); the tables are
+! built by the engine from these values
+   subroutine setphi(ndim,ntab,nbin,el,mass,hbin)
+   integer(kind=i4) :: ndim,ntab,nbin(:)
+   real(kind=r8) :: el,mass(:),hbin
+   read (5,*) prm%v0
+   read (5,*) prm%amu
+   read (5,*) prm%cjas1
+   read (5,*) prm%cjas2
+   read (5,*) prm%djas
+   read (5,*) prm%v0op
+   read (5,*) prm%v0eq
+   if (prm%cjas2.ne.0.0_r8) error stop 'setphi: equal-spin Jastrow (cjas(2) /= 0) is not supported'
+   end subroutine setphi
+end module potential
+
+module stack
+   use engine
+   implicit none
+   type walker
+      real(kind=r8), allocatable :: x(:,:),dpsi(:,:)
+      real(kind=r8) :: psil=0.0_r8,d2psi=0.0_r8,v=0.0_r8,weight=1.0_r8,wt0=1.0_r8
+      integer(kind=i4) :: is=1
+      integer(kind=i8) :: irn=0
+   end type walker
+   type(walker), save :: w1,w2
+   integer(kind=i4), private, save :: npart,ndim,cap
+! host side of the stacks: walkers pushed before the next device call, or downloaded for
+! popping.  Stack numbers select nothing: the device holds the one population.
+   real(kind=r8), allocatable, target, private, save :: hx(:,:,:),hd(:,:,:),hsc(:,:)
+   integer(kind=i8), allocatable, target, private, save :: hirn(:)
+   integer(kind=i4), private, save :: nhost=0
+   logical, private, save :: downloaded=.false.
+contains
+   subroutine create(nst,nw,np,nd,euclid)
+   integer(kind=i4) :: nst,nw,np,nd
+   logical :: euclid
+   npart=np
+   ndim=nd
+   cap=nw
+   allocate(hx(nd,np,nw),hd(nd,np,nw),hsc(6,nw),hirn(nw))
+   allocate(w1%x(nd,np),w1%dpsi(nd,np),w2%x(nd,np),w2%dpsi(nd,np))
+   nhost=0
+   end subroutine create
+
+! push (stack.f90:83-95): the walker joins the population at the next device call
+   subroutine push(i,w)
+   integer(kind=i4) :: i
+   type(walker) :: w
+   if (downloaded) then
+      nhost=0
+      downloaded=.false.
+   endif
+   if (nhost.ge.cap) error stop 'stack overflow'
+   nhost=nhost+1
+   hx(:,:,nhost)=w%x
+   hirn(nhost)=w%irn
+   end subroutine push
+
+! pop (stack.f90:97-112): the device population is downloaded on the first pop
+   subroutine pop(i,w,empty)
+   integer(kind=i4) :: i
+   type(walker) :: w
+   logical :: empty
+   integer(c_int32_t) :: nw
+   if (.not.downloaded.and.nhost.eq.0.and.started) then
+      call dmc_check(dmc_read_walkers(ctx,cap,nw,c_loc(hx),c_loc(hd),c_loc(hsc),c_loc(hirn)), &
+         'pop')
+      nhost=nw
+      downloaded=.true.
+      call dmc_check(dmc_load_walkers(ctx,0_c_int32_t,hx,hirn),'pop')
+   endif
+   empty=nhost.eq.0
+   if (empty) return
+   if (.not.allocated(w%x)) allocate(w%x(ndim,npart),w%dpsi(ndim,npart))
+   w%x=hx(:,:,nhost)
+   w%irn=hirn(nhost)
+   if (downloaded) then
+      w%dpsi=hd(:,:,nhost)
+      w%weight=hsc(1,nhost)
+      w%psil=hsc(2,nhost)
+      w%d2psi=hsc(3,nhost)
+      w%v=hsc(4,nhost)
+      w%is=nint(hsc(5,nhost))
+      w%wt0=hsc(6,nhost)
+   endif
+   nhost=nhost-1
+   end subroutine pop
+
+! walkers pushed on the host are loaded onto the device (initial hpsi there)
+   subroutine flush_stack
+   type(c_ptr) :: c
+   c=engine_ctx()
+   if (nhost.gt.0.and..not.downloaded) then
+      call dmc_check(dmc_load_walkers(c,nhost,hx,hirn),'load walkers')
+      nhost=0
+   endif
+   end subroutine flush_stack
+
+   function numstack(i)
+   integer(kind=i4) :: numstack,i
+   integer(c_int32_t) :: nw
+   nw=0
+   if (started) call dmc_check(dmc_population(ctx,nw),'numstack')
+   numstack=nw+nhost
+   end function numstack
+end module stack
+
+module wavefunction
+   use engine
+   implicit none
+   integer(kind=i4), private, save :: npart,norb
+contains
+! setpsi: nodes, a2 (wavefunction.f90:36-37), then the pairing deck (psislaterbcs.f90:47-68)
+! and the beta deck (betafun.f90:304-306); returns the box side el
+   subroutine setpsi(ndim,nbin,el,mass,ntab)
+   integer(kind=i4) :: ndim,nbin(:),ntab
+   real(kind=r8) :: el,mass(:)
+   character(len=8) :: nodes
+   real(kind=r8) :: a2,v0ex,v0exjas,jbeta,jgamma
+   integer(kind=i4) :: nexup,nexdn,numq,i,nqex(3)
+   read (5,'(a8)') nodes
+   read (5,*) a2
+   if (trim(adjustl(nodes)).ne.'bcs') error stop 'setpsi: only bcs nodes are supported'
+   if (a2.ne.0.0_r8) error stop 'setpsi: a guiding function (a2 /= 0) is not supported'
+   read (5,*) prm%rho
+   read (5,*) nexup
+   if (nexup.ne.0) error stop 'setpsi: unpaired up orbitals are not supported'
+   read (5,*) nexdn
+   if (nexdn.ne.0) error stop 'setpsi: unpaired down orbitals are not supported'
+   read (5,*) norb
+   if (norb.gt.16) error stop 'setpsi: more than 16 shells'
+   prm%norb=norb
+   prm%vorb=0.0_r8
+   do i=1,norb
+      read (5,*) prm%vorb(i)
+   enddo
+   read (5,*) nqex
+   read (5,*) v0ex
+   read (5,*) v0exjas
+   read (5,*) numq
+   read (5,*) jbeta
+   read (5,*) jgamma
+   if (any(nqex.ne.0).or.v0ex.ne.0.0_r8.or.v0exjas.ne.0.0_r8.or.numq.ne.0) &
+      error stop 'setpsi: external potential and long-range Jastrow are not supported'
+   read (5,*) prm%bbeta
+   read (5,*) prm%dbeta
+   read (5,*) prm%bfact
+   if (prm%bfact.lt.0.0_r8) error stop 'setpsi: beta tables from file (bfact < 0) are not supported'
+   npart=sum(nbin)
+   prm%npart=npart
+   prm%ntab=ntab
+   prm%mass=mass(1)
+   if (nbin(1).ne.nbin(2).or.mass(1).ne.mass(2)) error stop 'setpsi: unpolarised equal masses only'
+   el=(npart/prm%rho)**(1.0_r8/ndim)
+   end subroutine setpsi
+
+! hpsi (wavefunction.f90:161-201) of one walker, on the device
+   subroutine hpsi(w)
+   use stack, only : walker
+   type(walker) :: w
+   real(kind=r8) :: psil(1),d2psi(1),v(1)
+   integer(kind=i4) :: is(1)
+   call dmc_check(dmc_eval(engine_ctx(),1_c_int32_t,w%x,psil,is,w%dpsi,d2psi,v),'hpsi')
+   w%psil=psil(1)
+   w%d2psi=d2psi(1)
+   w%v=v(1)
+   w%is=is(1)
+   end subroutine hpsi
+
+! getderpsi (wavefunction.f90:203-229): d log psi / d(alpha_1..alpha_norb[, b, d]) at x
+   subroutine getderpsi(x,dpsi)
+   real(kind=r8) :: x(:,:),dpsi(:)
+   real(kind=r8) :: d(norb+2)
+   call dmc_check(dmc_adjoint(engine_ctx(),1_c_int32_t,x,d,c_null_ptr),'getderpsi')
+   dpsi=d(1:size(dpsi))
+   end subroutine getderpsi
+
+! d log psi and d E_L with respect to the parameters for nw configurations x(3,npart,nw)
+   subroutine getderelocal(nw,x,dlogpsi,delocal)
+   integer(kind=i4) :: nw
+   real(kind=r8), target :: x(:,:,:),dlogpsi(:,:),delocal(:,:)
+   call dmc_check(dmc_adjoint(engine_ctx(),nw,x,dlogpsi,c_loc(delocal)),'getderelocal')
+   end subroutine getderelocal
+
+! setdetparam (wavefunction.f90:231-244): alpha_1..alpha_norb[, b, d]
+   subroutine setdetparam(par)
+   real(kind=r8) :: par(:)
+   prm%vorb(1:norb)=par(1:norb)
+   if (size(par).ge.norb+2) then
+      prm%bbeta=par(norb+1)
+      prm%dbeta=par(norb+2)
+   endif
+   if (started) call dmc_check(dmc_set_params(ctx,prm%vorb,prm%bbeta,prm%dbeta),'setdetparam')
+   end subroutine setdetparam
+
+   subroutine getdetparam(npar,par)
+   integer(kind=i4) :: npar
+   real(kind=r8) :: par(:)
+   npar=norb
+   par(1:norb)=prm%vorb(1:norb)
+   if (size(par).ge.norb+2) then
+      npar=norb+2
+      par(norb+1)=prm%bbeta
+      par(norb+2)=prm%dbeta
+   endif
+   end subroutine getdetparam
+
+! move1 (wavefunction.f90:393-404): psi(x with particle i at xnew)/psi(x)
+   subroutine move1(i,xnew,w,psirat)
+   use stack, only : walker
+   integer(kind=i4) :: i
+   real(kind=r8) :: xnew(:),psirat
+   type(walker) :: w
+   integer(kind=i4) :: ip(1)
+   real(kind=r8) :: r(1)
+   ip(1)=i-1
+   call dmc_check(dmc_ratio1(engine_ctx(),1_c_int32_t,w%x,ip,xnew,r),'move1')
+   psirat=r(1)
+   end subroutine move1
+end module wavefunction
+
+module step
+   use engine
+   use stack, only : flush_stack
+   implicit none
+   integer(kind=i4), private, save :: idmc
+contains
+! setstep (step.f90:11-41).  The engine derives sigma, the drift cut and the internal trial
+! energy from the deck values itself; the scaled etrial and eunit passed here are not used.
+   subroutine setstep(idmcin,ndim,npart,nbin,hbar,dt,etrial,el,mass,wtcut,dcut,eunit,fn)
+   integer(kind=i4) :: idmcin,ndim,npart,nbin(:)
+   real(kind=r8) :: hbar,dt,etrial,el,mass(:),wtcut,dcut,eunit
+   logical :: fn
+   idmc=idmcin
+   if (idmc.ne.1) error stop 'setstep: the engine propagates DMC (idmc = 1)'
+   prm%hbar=hbar
+   prm%dt=dt
+   prm%wtcut=wtcut
+   prm%dcut=dcut
+   prm%fn=merge(1,0,fn)
+   end subroutine setstep
+
+! step1 (step.f90:43-285): the whole population is propagated on the device
+   subroutine step1(istin,istout,opcomp)
+   integer(kind=i4) :: istin,istout
+   logical :: opcomp
+   call flush_stack
+   call dmc_check(dmc_propagate(engine_ctx(),merge(1,0,opcomp)),'step1')
+   end subroutine step1
+
+! branch (step.f90:287-315)
+   subroutine branch(istin,istout,factor)
+   integer(kind=i4) :: istin,istout
+   real(kind=r8) :: factor
+   integer(c_int32_t) :: nt
+   call flush_stack
+   call dmc_check(dmc_branch(engine_ctx(),factor,nt),'branch')
+   end subroutine branch
+
+! writewtisto (step.f90:317-334): histogram of branch multiplicities
+   subroutine writewtisto
+   integer(c_int64_t) :: hist(500)
+   integer(kind=i4) :: i
+   call dmc_check(dmc_wtisto(engine_ctx(),hist),'writewtisto')
+   open (unit=33,file='wtisto',status='replace')
+   do i=1,500
+      if (hist(i).gt.0) write (33,'(i5,i14)') i-1,hist(i)
+   enddo
+   close(33)
+   end subroutine writewtisto
+end module step
+
+module estimator
+   use engine
+   implicit none
+   integer(kind=i4), private, save :: nest=0
+   character(len=30), allocatable, private, save :: label(:)
+   real(kind=r8), allocatable, private, save :: valnow(:),errnow(:)
+contains
+   subroutine setestnum(n)
+   integer(kind=i4) :: n
+   nest=n
+   allocate(label(n),valnow(n),errnow(n))
+   label=' '
+   valnow=0.0_r8
+   errnow=0.0_r8
+   end subroutine setestnum
+
+   subroutine addest(i,l)
+   integer(kind=i4) :: i
+   character(len=*) :: l
+   label(i)=l
+   end subroutine addest
+
+! zerest: discard the sums accumulated so far (end of equilibration)
+   subroutine zerest
+   real(c_double) :: est(3,14)
+   call dmc_check(dmc_estimators(engine_ctx(),est,1_c_int32_t),'zerest')
+   end subroutine zerest
+
+! update (estimator.f90:84-112): block value and its error from the device block sums
+! (sum val*wt, sum val^2*wt, sum wt), then the sums are cleared
+   subroutine update
+   real(c_double) :: est(3,14)
+   integer(kind=i4) :: i
+   real(kind=r8) :: v2
+   call dmc_check(dmc_estimators(engine_ctx(),est,1_c_int32_t),'update')
+   do i=1,nest
+      valnow(i)=est(1,i)/est(3,i)
+      v2=est(2,i)/est(3,i)
+      errnow(i)=sqrt(abs(v2-valnow(i)**2)/est(3,i))
+   enddo
+   end subroutine update
+
+! resstring (estimator.f90:135-149): one line per estimator, block value +- block error
+   function resstring(tau)
+   real(kind=r8) :: tau
+   character(len=90), dimension(nest) :: resstring
+   integer(kind=i4) :: i
+   do i=1,nest
+      write (resstring(i),'(a30,t30,1x,1p,e15.8,e19.10,'' +- '',e19.10)') &
+         label(i),tau,valnow(i),errnow(i)
+   enddo
+   end function resstring
+end module estimator

@@ -53,6 +53,8 @@ SIGNATURES = {
     "dmc_eval": [_vp, C.c_int32, _f64, _f64, _i32, _f64, _f64, _f64],
     "dmc_ratio1": [_vp, C.c_int32, _f64, _i32, _f64, _f64],
     "dmc_step": [_vp, C.c_int32, _i32],
+    "dmc_propagate": [_vp, C.c_int32],
+    "dmc_branch": [_vp, C.c_double, C.POINTER(C.c_int32)],
     "dmc_run": [_vp, C.c_int32, C.POINTER(C.c_int32)],
     "dmc_estimators": [_vp, _vp, C.c_int32],
     "dmc_wtisto": [_vp, _i64],

@@ -100,6 +100,15 @@ int dmc_ratio1(dmc_ctx *c, int32_t nw, const double *x, const int32_t *ip,
  * accumulates the energy estimators as step1 does on the last step of a block.
  * nt[0] = population after branching, nt[1] = after population control. */
 int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt);
+/* The two halves of dmc_step, for a host that keeps the reference's loop
+ * (dmcbcs.f90:437-445):
+ * dmc_propagate replaces step1 (step.f90:43-285): moves, hpsi, weights, estimators and the
+ * branch draw of every walker; the population is not changed until dmc_branch.
+ * dmc_branch replaces branch (step.f90:287-315).  Right after dmc_propagate it takes
+ * factor 1 and applies the drawn multiplicities; otherwise it is a population-control pass
+ * with multiplicities int(weight*factor + u).  nt = the population afterwards. */
+int dmc_propagate(dmc_ctx *c, int32_t opcomp);
+int dmc_branch(dmc_ctx *c, double factor, int32_t *nt);
 /* Run nsteps steps (opcomp on the last), returning the final population. */
 int dmc_run(dmc_ctx *c, int32_t nsteps, int32_t *nt);
 /* Estimator accumulators est[14][3] = (sum val*wt, sum val^2*wt, sum wt) in the order of
