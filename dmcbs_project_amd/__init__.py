@@ -5,7 +5,7 @@ include/dmc_gpu.h) does the work; this package is a thin host-side binding whose
 mirror the reference's Fortran module interfaces (step, stack, wavefunction).
 """
 from .lib import DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMCError, load
-from .api import Params, Engine
+from .api import Params, Engine, Batch
 
-__all__ = ["Params", "Engine", "DMCError", "load", "DMC_FP64", "DMC_MIXED", "DMC_ALLPARTICLE",
+__all__ = ["Params", "Engine", "Batch", "DMCError", "load", "DMC_FP64", "DMC_MIXED", "DMC_ALLPARTICLE",
            "DMC_SINGLEPARTICLE"]

@@ -215,3 +215,48 @@ class Engine:
 
     def sync(self):
         L.check(self.lib.dmc_sync(self.h))
+
+
+class Batch:
+    """Several engines (different particle numbers) stepped together by the device: one
+    captured graph per step with one persistent launch per phase over all systems
+    (dmc_batch_*).  Steps are asynchronous; sync() returns the populations."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        self.lib = L.load()
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        L.check(self.lib.dmc_batch_create(arr, len(self.engines), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, opcomp=False):
+        L.check(self.lib.dmc_batch_step(self.h, int(bool(opcomp))))
+
+    def run(self, nsteps):
+        L.check(self.lib.dmc_batch_run(self.h, int(nsteps)))
+
+    def sync(self):
+        nt = np.zeros(len(self.engines), np.int32)
+        L.check(self.lib.dmc_batch_sync(self.h, nt))
+        return [int(v) for v in nt]
+
+    def set_wpg(self, moves=1, refresh=1, end=1):
+        L.check(self.lib.dmc_batch_set_wpg(self.h, int(moves), int(refresh), int(end)))
+
+    def tune(self, reps=3):
+        """Walkers per workgroup of every phase from timed launches; returns the choice and
+        the ms per launch [phase][wpg = 1, 2, 4, 8] (-1: not run)."""
+        w = np.zeros(3, np.int32)
+        ms = np.zeros(12)
+        L.check(self.lib.dmc_batch_tune(self.h, int(reps), w, ms))
+        return dict(zip(("moves", "refresh", "end"), (int(v) for v in w))), ms.reshape(3, 4)
+

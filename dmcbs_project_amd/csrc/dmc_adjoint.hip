@@ -52,7 +52,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
     double *res = sm + X.res;
     for (int t = tid; t < 3 * npar; t += 64) res[t] = 0.0;
-    __syncthreads();
+    wave_sync();
     const Tile<NUP> T(tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
     const double *xs = sm + L.xs, *sinv = sm + L.sinv, *qc = sm + L.qc;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
                 if (T.rv[a]) pj[(d * NUP + T.ri[a]) * 8 + c] = ju[a][d];
                 if (T.cv[a]) pj[((3 + d) * NUP + T.cj[a]) * 8 + r] = jd[a][d];
             }
-        __syncthreads();
+        wave_sync();
         for (int p = tid; p < NP; p += 64) {
             const int off = p < NUP ? 0 : 3, pp = p < NUP ? p : p - NUP;
 #pragma unroll
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
                 du[3 * p + d] = s;
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 
     const double cL = 2.0 * S.invm, cG = 2.0 * S.invsqrtm;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
                 if (T.rv[a] && T.cv[b]) hm[T.ri[a] * SP + T.cj[b]] = h;
             }
     }
-    __syncthreads();
+    wave_sync();
     // T = H Sinv (up x up), then KT(i, j) = K(j, i) = sum_m Sinv(j, m) T(m, i)
 #pragma unroll
     for (int a = 0; a < TR; ++a)
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
             for (int j = 0; j < NUP; ++j) s += hm[T.ri[a] * SP + j] * sinv[j * SP + T.cj[b]];
             if (T.rv[a] && T.cv[b]) tm[T.ri[a] * SP + T.cj[b]] = s;  // T(i = row a, k = col b)
         }
-    __syncthreads();
+    wave_sync();
     // K(j, i) of the tile into the H area (H is consumed), the inverse tile into registers
     double *km = hm;
     double sv[TR][TR];
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
             sv[a][b] = ok ? sinv[j * SP + i] : 0.0;
             if (T.rv[a] && T.cv[b]) km[i * SP + j] = ok ? s : 0.0;
         }
-    __syncthreads();
+    wave_sync();
     auto kt = [&](int a, int b) {
         return (T.rv[a] && T.cv[b]) ? km[T.ri[a] * SP + T.cj[b]] : 0.0;
     };
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
             res[2 * npar + sh] += 2.0 * bs;
         }
     }
-    __syncthreads();
+    wave_sync();
     for (int p = tid; p < npar; p += 64) {
         const bool ok = !D.singular;
         dlog[size_t(w) * npar + p] = ok ? res[p] : 0.0;

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict_
     constexpr int M = 6 * NUP;
     load_q<NUP>(S, sm, L, tid);
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
-    __syncthreads();
+    wave_sync();
     const Tile<NUP> T(tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
     if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_bytes(S), tid);
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_step(DevSys S, DevState A, DevState B, D
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     hpsi_block<NUP>(S, sm, L, tid);
     // new RNG state after gaussian(M)
     const uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(64) void k_ratio(DevSys S, const double *__restrict
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
     double *xn = sm + L.xn;
     if (tid < 3) xn[tid] = xnew[3 * w + tid];
-    __syncthreads();
+    wave_sync();
     const Tile<NUP> T(tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
     const int i = ip[w];
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(64) void k_ratio(DevSys S, const double *__restrict
     double2 *phn = reinterpret_cast<double2 *>(sm + L.xn + 4);
     const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
     if (tid == 0) phases_of(S, qm, xn, phn, 1);
-    __syncthreads();
+    wave_sync();
     const double *qc = sm + L.qc, *sinv = sm + L.sinv, *xs = sm + L.xs;
     double part = 0.0;
     if (tid < NUP) {

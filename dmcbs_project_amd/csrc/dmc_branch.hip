@@ -12,10 +12,19 @@ namespace dmc {
 
 constexpr int kScanThreads = 1024;
 
+// Device population words of DevStep::nt: [0] walkers a kernel processes, [1] total after a
+// scan, [2] resident population (device-driven steps), [3] overflow flag, [4] population
+// control active.  A non-null gate makes a kernel return at once unless *gate is set, so
+// that a captured graph can hold the population-control pass unconditionally.
+__device__ __forceinline__ bool gated_off(const int *gate) { return gate && !*gate; }
+
 // multiplicities of state A with a factor (population control, dmcbcs.f90:445)
-__global__ void k_branch_count(DevState A, DevStep T, double factor) {
+__global__ void k_branch_count(DevState A, DevStep T, double factor, const int *gate, int nwalk) {
+    if (gated_off(gate)) return;
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= T.nt[0]) return;
+    // gated (device-driven) pass: the factor nwalk/nt of dmcbcs.f90:445 from the device count
+    if (gate) factor = 1.0 * nwalk / T.nt[2];
     const double wt = A.weight[w] * factor;
     uint64_t s = A.irn[w];
     s = (s * kM1 + kA1) & kMask48;
@@ -27,7 +36,8 @@ __global__ void k_branch_count(DevState A, DevStep T, double factor) {
 }
 
 // exclusive scan of count[0..nt[0]) into offset, total into nt[1]; one block
-__global__ __launch_bounds__(kScanThreads) void k_scan(DevStep T) {
+__global__ __launch_bounds__(kScanThreads) void k_scan(DevStep T, const int *gate) {
+    if (gated_off(gate)) return;
     __shared__ int part[kScanThreads];
     const int n = T.nt[0];
     const int tid = threadIdx.x;
@@ -52,7 +62,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevStep T) {
 }
 
 // destination map and the RNG state of every copy
-__global__ void k_branch_map(DevState src, DevStep T, int capacity) {
+__global__ void k_branch_map(DevState src, DevStep T, int capacity, const int *gate) {
+    if (gated_off(gate)) return;
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= T.nt[0]) return;
     const int off = T.offset[w], cnt = T.count[w];
@@ -68,9 +79,10 @@ __global__ void k_branch_map(DevState src, DevStep T, int capacity) {
 
 // copy every destination walker from its source; weight becomes sign(1, weight)
 __global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState dst, DevStep T,
-                                               int m3) {
+                                               int m3, const int *gate, int capacity) {
+    if (gated_off(gate)) return;
     const int d = blockIdx.x;
-    if (d >= T.nt[1]) return;
+    if (d >= T.nt[1] || d >= capacity) return;
     const int w = T.map[d];
     const double *xs = src.x + size_t(w) * m3, *gs = src.dpsi + size_t(w) * m3;
     double *xd = dst.x + size_t(d) * m3, *gd = dst.dpsi + size_t(d) * m3;
@@ -95,10 +107,12 @@ __global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState 
     }
 }
 
-// est_acc[k] += sum_w est[w][k], k < 42, in a fixed order (one block per component)
+// est_acc[k] += sum_w est[w][k], k < 42, in a fixed order (one block per component);
+// nw < 0: the count in nt[0]
 __global__ __launch_bounds__(256) void k_est_reduce(DevStep T, int nw, double *acc) {
     __shared__ double part[256];
     const int k = blockIdx.x, tid = threadIdx.x;
+    if (nw < 0) nw = T.nt[0];
     double s = 0.0;
     for (int w = tid; w < nw; w += 256) s += T.est[size_t(w) * 42 + k];
     part[tid] = s;
@@ -110,24 +124,110 @@ __global__ __launch_bounds__(256) void k_est_reduce(DevStep T, int nw, double *a
     if (tid == 0) acc[k] += part[0];
 }
 
+// device-driven population bookkeeping after a branch pass: resident population
+// nt[2] = min(total, capacity), overflow flag nt[3]; then (decide != 0) the population-
+// control test of dmcbcs.f90:444-445, setting nt[4] and, when it fires, nt[0] = nt[2]
+__global__ void k_pop_update(DevStep T, int capacity, double lo, double hi, int decide,
+                             const int *gate) {
+    if (gated_off(gate)) return;
+    const int total = T.nt[1];
+    const int n = total < capacity ? total : capacity;
+    T.nt[2] = n;
+    if (total > capacity) T.nt[3] = 1;
+    if (decide) {
+        const int on = (n > hi || n < lo) && n > 0;
+        T.nt[4] = on;
+        if (on) T.nt[0] = n;
+    }
+}
+
+// start of a device-driven step: the kernels process the resident population
+__global__ void k_pop_begin(DevStep T) {
+    T.nt[0] = T.nt[2];
+}
+
+// copy walkers 0..nt[1]-1 of src to dst (the population-control result back to A)
+__global__ __launch_bounds__(64) void k_copy_back(DevSys S, DevState src, DevState dst,
+                                                  DevStep T, int m3, const int *gate,
+                                                  int capacity) {
+    if (gated_off(gate)) return;
+    const int d = blockIdx.x;
+    if (d >= T.nt[1] || d >= capacity) return;
+    const double *xs = src.x + size_t(d) * m3, *gs = src.dpsi + size_t(d) * m3;
+    double *xd = dst.x + size_t(d) * m3, *gd = dst.dpsi + size_t(d) * m3;
+    for (int t = threadIdx.x; t < m3; t += 64) {
+        xd[t] = xs[t];
+        gd[t] = gs[t];
+    }
+    if (S.keep_sinv) {
+        const size_t nb = sinv_bytes(S) / 4;
+        const uint32_t *a = reinterpret_cast<const uint32_t *>(src.sinv + size_t(d) * nb * 4);
+        uint32_t *b = reinterpret_cast<uint32_t *>(dst.sinv + size_t(d) * nb * 4);
+        for (size_t t = threadIdx.x; t < nb; t += 64) b[t] = a[t];
+    }
+    if (threadIdx.x == 0) {
+        dst.psil[d] = src.psil[d];
+        dst.d2psi[d] = src.d2psi[d];
+        dst.v[d] = src.v[d];
+        dst.is[d] = src.is[d];
+        dst.wt0[d] = src.wt0[d];
+        dst.weight[d] = src.weight[d];
+        dst.irn[d] = src.irn[d];
+    }
+}
+
 int launch_branch_count(const DevSys &, const DevState &A, const DevStep &T, double factor,
                         int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_branch_count, dim3((grid + 255) / 256), dim3(256), 0, st, A, T, factor);
+    hipLaunchKernelGGL(k_branch_count, dim3((grid + 255) / 256), dim3(256), 0, st, A, T, factor,
+                       nullptr, 0);
     return hipGetLastError();
 }
 
 // scan, then (after the caller has read nt[1]) map + gather: split in two launchers
 int launch_scan(const DevStep &T, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, T);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, T, nullptr);
     return hipGetLastError();
 }
 
 int launch_map_gather(const DevSys &S, const DevState &src, const DevState &dst,
                       const DevStep &T, int nsrc, int ndst, int capacity, hipStream_t st) {
-    hipLaunchKernelGGL(k_branch_map, dim3((nsrc + 255) / 256), dim3(256), 0, st, src, T, capacity);
+    hipLaunchKernelGGL(k_branch_map, dim3((nsrc + 255) / 256), dim3(256), 0, st, src, T, capacity,
+                       nullptr);
     int e = hipGetLastError();
     if (e) return e;
-    if (ndst > 0) hipLaunchKernelGGL(k_gather, dim3(ndst), dim3(64), 0, st, S, src, dst, T, 3 * S.npart);
+    if (ndst > 0)
+        hipLaunchKernelGGL(k_gather, dim3(ndst), dim3(64), 0, st, S, src, dst, T, 3 * S.npart,
+                           nullptr, capacity);
+    return hipGetLastError();
+}
+
+// The branch and population control of one system with every size on the device (no host
+// synchronisation; capturable in a graph).  The population enters in B (after the step
+// kernels) with multiplicities in T.count and leaves in A; population control, when it
+// fires, runs A -> B and copies back.
+int launch_branch_device(const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
+                         int capacity, hipStream_t st) {
+    const int m3 = 3 * S.npart, g256 = (capacity + 255) / 256;
+    const int *gate = T.nt + 4;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, T, nullptr);
+    hipLaunchKernelGGL(k_branch_map, dim3(g256), dim3(256), 0, st, B, T, capacity, nullptr);
+    hipLaunchKernelGGL(k_gather, dim3(capacity), dim3(64), 0, st, S, B, A, T, m3, nullptr,
+                       capacity);
+    hipLaunchKernelGGL(k_pop_update, dim3(1), dim3(1), 0, st, T, capacity,
+                       S.popdn * S.nwalk, S.popup * S.nwalk, 1, nullptr);
+    hipLaunchKernelGGL(k_branch_count, dim3(g256), dim3(256), 0, st, A, T, 1.0, gate, S.nwalk);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, T, gate);
+    hipLaunchKernelGGL(k_branch_map, dim3(g256), dim3(256), 0, st, A, T, capacity, gate);
+    hipLaunchKernelGGL(k_gather, dim3(capacity), dim3(64), 0, st, S, A, B, T, m3, gate,
+                       capacity);
+    hipLaunchKernelGGL(k_copy_back, dim3(capacity), dim3(64), 0, st, S, B, A, T, m3, gate,
+                       capacity);
+    hipLaunchKernelGGL(k_pop_update, dim3(1), dim3(1), 0, st, T, capacity, 0.0, 0.0, 0, gate);
+    return hipGetLastError();
+}
+
+int launch_pop_begin(const DevStep &T, hipStream_t st) {
+    hipLaunchKernelGGL(k_pop_begin, dim3(1), dim3(1), 0, st, T);
     return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 // population control).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -10,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "dmc_batch.h"
 #include "dmc_internal.h"
 #include "dmc_wave.h"
 
@@ -285,7 +287,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         c->alloc_state(c->A) || c->alloc_state(c->B) || c->alloc(&c->T.count, c->capacity) ||
         c->alloc(&c->T.offset, c->capacity) || c->alloc(&c->T.map, c->capacity) ||
         c->alloc(&c->T.copy_irn, c->capacity) || c->alloc(&c->T.est, size_t(c->capacity) * 42) ||
-        c->alloc(&c->T.nt, 4) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
+        c->alloc(&c->T.nt, 8) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
         c->alloc(&c->T.moves, 2) ||
         c->alloc(&c->est_acc, 42)) {
         std::string e = g_err;
@@ -670,6 +672,331 @@ int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double
     if (delocal)
         HIPCHK(hipMemcpyAsync(delocal, de, sizeof(double) * np * nw, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Batches: several contexts stepped together, one captured graph per step
+
+struct dmc_batch {
+    std::vector<dmc_ctx *> ctx;     // in the caller's order
+    std::vector<int> order;         // launch order (largest particle number first)
+    BatchHost host;
+    BatchArgs args{};
+    hipStream_t st = nullptr;
+    int *counters = nullptr;        // work counters, kBatchCounters
+    int ncu = 1, nchunk = 1;
+    int wpg[3] = {1, 1, 1};         // walkers per workgroup: moves, refresh, end
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool dirty = true, host_current = true;
+    std::vector<char *> captured_a;  // A.x of every system when the graphs were captured
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    static constexpr int kBatchCounters = 64 * kMaxBatch;
+};
+
+namespace {
+
+void batch_drop_graphs(dmc_batch *b) {
+    for (int i = 0; i < 2; ++i) {
+        if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+        if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
+        b->exec[i] = nullptr;
+        b->graph[i] = nullptr;
+    }
+    b->dirty = true;
+}
+
+// the launches of one batch step on b->st (captured, or run directly by the tuner)
+int batch_enqueue(dmc_batch *b, int opcomp) {
+    const int ns = int(b->order.size());
+    LAUNCHCHK(launch_batch_begin(b->args, b->counters, dmc_batch::kBatchCounters, b->st));
+    int slot = 0;
+    for (int c = 0; c < b->nchunk; ++c) {
+        if (c > 0)
+            LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::Refresh, c, 0,
+                                         b->counters + kMaxBatch * slot++, b->wpg[1], b->ncu, b->st));
+        LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::Moves, c, 0,
+                                     b->counters + kMaxBatch * slot++, b->wpg[0], b->ncu, b->st));
+    }
+    LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::End, -1, opcomp,
+                                 b->counters + kMaxBatch * slot++, b->wpg[2], b->ncu, b->st));
+    for (int i = 0; i < ns; ++i) {
+        dmc_ctx *c = b->ctx[b->order[i]];
+        LAUNCHCHK(launch_est_reduce(c->T, -1, c->est_acc, b->st));
+        LAUNCHCHK(launch_branch_device(c->ds, c->A, c->B, c->T, c->capacity, b->st));
+    }
+    return DMC_OK;
+}
+
+// refresh the system records from the contexts (their A/B may have been swapped by steps
+// outside the batch) and capture the two step graphs (opcomp 0 and 1)
+int batch_capture(dmc_batch *b) {
+    batch_drop_graphs(b);
+    const int ns = int(b->order.size());
+    b->args.nsys = ns;
+    b->captured_a.assign(ns, nullptr);
+    for (int i = 0; i < ns; ++i) {
+        dmc_ctx *c = b->ctx[b->order[i]];
+        BatchSys &q = b->host.sys[i];
+        q.S = c->ds;
+        q.A = c->A;
+        q.B = c->B;
+        q.T = c->T;
+        b->args.sys[i] = q;
+        b->captured_a[i] = reinterpret_cast<char *>(c->A.x);
+    }
+    for (int oc = 0; oc < 2; ++oc) {
+        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int e = batch_enqueue(b, oc);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(b->st, &g);
+        if (e) {
+            if (g) (void)hipGraphDestroy(g);
+            return e;
+        }
+        if (ee != hipSuccess) return fail(DMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+        b->graph[oc] = g;
+        HIPCHK(hipGraphInstantiate(&b->exec[oc], g, nullptr, nullptr, 0));
+    }
+    b->dirty = false;
+    return DMC_OK;
+}
+
+// host populations into the device words (nt[2]) of every system
+int batch_upload_populations(dmc_batch *b) {
+    for (dmc_ctx *c : b->ctx) {
+        if (c->pending) return fail(DMC_ERR_ARG, "a member context awaits dmc_branch");
+        int h[5] = {c->nw, 0, c->nw, 0, 0};
+        HIPCHK(hipMemcpyAsync(c->T.nt, h, sizeof(h), hipMemcpyHostToDevice, b->st));
+    }
+    HIPCHK(hipStreamSynchronize(b->st));
+    return DMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmc_batch_create(dmc_ctx *const *ctxs, int32_t n, dmc_batch **out) {
+    if (!ctxs || !out || n < 1) return fail(DMC_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    if (n > kMaxBatch) return fail(DMC_ERR_ARG, "a batch holds at most " + std::to_string(kMaxBatch) + " systems");
+    for (int i = 0; i < n; ++i) {
+        const dmc_ctx *c = ctxs[i];
+        if (!c) return fail(DMC_ERR_ARG, "null context");
+        if (c->sys.p.movetype != DMC_SINGLEPARTICLE)
+            return fail(DMC_ERR_ARG, "batches run single-particle moves");
+        if (c->sys.p.precision != ctxs[0]->sys.p.precision)
+            return fail(DMC_ERR_ARG, "the systems of a batch share one precision");
+        if (c->nw <= 0) return fail(DMC_ERR_ARG, "every system needs walkers");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == c) return fail(DMC_ERR_ARG, "a context appears twice");
+    }
+    dmc_batch *b = new dmc_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->order.resize(n);
+    for (int i = 0; i < n; ++i) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+        return b->ctx[x]->sys.npart > b->ctx[y]->sys.npart;
+    });
+    b->host.fp32 = ctxs[0]->sys.p.precision == DMC_MIXED;
+    b->host.sys.resize(n);
+    b->nchunk = 1;
+    for (int i = 0; i < n; ++i) {
+        const dmc_ctx *c = b->ctx[b->order[i]];
+        BatchSys &q = b->host.sys[i];
+        q.nup = c->sys.n;
+        q.npart = c->sys.npart;
+        q.k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : c->sys.npart;
+        q.capacity = c->capacity;
+        q.S = c->ds;
+        const int nc = (q.npart + q.k - 1) / q.k;
+        b->nchunk = nc > b->nchunk ? nc : b->nchunk;
+    }
+    if (2 * b->nchunk + 1 > 64) {
+        delete b;
+        return fail(DMC_ERR_ARG, "too many refresh chunks for a batch");
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipStreamCreate(&b->st) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&b->counters), sizeof(int) * dmc_batch::kBatchCounters) != hipSuccess ||
+        hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+        dmc_batch_destroy(b);
+        return fail(DMC_ERR_HIP, "batch set-up failed");
+    }
+    for (dmc_ctx *c : b->ctx) (void)hipStreamSynchronize(c->st);
+    int e = batch_upload_populations(b);
+    if (e) {
+        std::string m = g_err;
+        dmc_batch_destroy(b);
+        return fail(e, m);
+    }
+    *out = b;
+    return DMC_OK;
+}
+
+int dmc_batch_destroy(dmc_batch *b) {
+    if (!b) return DMC_OK;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    batch_drop_graphs(b);
+    if (b->counters) (void)hipFree(b->counters);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return DMC_OK;
+}
+
+int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end) {
+    if (!b) return fail(DMC_ERR_ARG, "null batch");
+    const int32_t w[3] = {moves, refresh, end};
+    const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
+    for (int i = 0; i < 3; ++i) {
+        if (w[i] != 1 && w[i] != 2 && w[i] != 4 && w[i] != 8)
+            return fail(DMC_ERR_ARG, "walkers per workgroup must be 1, 2, 4 or 8");
+        if (batch_blocks_per_cu(b->host, ph[i], w[i]) <= 0)
+            return fail(DMC_ERR_ARG, "walkers per workgroup " + std::to_string(w[i]) +
+                                         " does not fit the LDS or registers of a CU");
+    }
+    for (int i = 0; i < 3; ++i) b->wpg[i] = w[i];
+    b->dirty = true;
+    return DMC_OK;
+}
+
+int dmc_batch_step(dmc_batch *b, int32_t opcomp) {
+    if (!b) return fail(DMC_ERR_ARG, "null batch");
+    if (b->host_current) {
+        int e = batch_upload_populations(b);
+        if (e) return e;
+        b->host_current = false;
+    }
+    for (size_t i = 0; i < b->order.size() && !b->dirty; ++i)
+        if (reinterpret_cast<char *>(b->ctx[b->order[i]]->A.x) != b->captured_a[i]) b->dirty = true;
+    if (b->dirty) {
+        int e = batch_capture(b);
+        if (e) return e;
+    }
+    HIPCHK(hipGraphLaunch(b->exec[opcomp ? 1 : 0], b->st));
+    return DMC_OK;
+}
+
+int dmc_batch_run(dmc_batch *b, int32_t nsteps) {
+    for (int s = 0; s < nsteps; ++s) {
+        int e = dmc_batch_step(b, s == nsteps - 1);
+        if (e) return e;
+    }
+    return DMC_OK;
+}
+
+int dmc_batch_sync(dmc_batch *b, int32_t *nt) {
+    if (!b) return fail(DMC_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->st));
+    const int n = int(b->ctx.size());
+    int err = DMC_OK;
+    std::string msg;
+    for (int i = 0; i < n; ++i) {
+        dmc_ctx *c = b->ctx[i];
+        int h[5];
+        HIPCHK(hipMemcpy(h, c->T.nt, sizeof(h), hipMemcpyDeviceToHost));
+        c->nw = h[2];
+        if (nt) nt[i] = h[2];
+        if (h[3] && !err) {
+            err = DMC_ERR_OVERFLOW;
+            msg = "stack overflow: a population exceeded capacity " + std::to_string(c->capacity);
+        } else if (h[2] == 0 && !err) {
+            err = DMC_ERR_ARG;
+            msg = "population died out";
+        }
+    }
+    b->host_current = true;
+    if (err) return fail(err, msg);
+    return DMC_OK;
+}
+
+int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out) {
+    if (!b || reps < 1) return fail(DMC_ERR_ARG, "invalid argument");
+    int e = DMC_OK;
+    if (b->host_current) {
+        e = batch_upload_populations(b);
+        if (e) return e;
+        b->host_current = false;
+    }
+    HIPCHK(hipStreamSynchronize(b->st));
+    // the tuning launches leave the population alone (they read A and write B and scratch);
+    // what they do touch is saved and restored: acceptance counters, the histogram and, for
+    // the refresh phase, the working inverse
+    const int ns = int(b->order.size());
+    std::vector<std::vector<char>> save(ns * 3);
+    auto snap = [&](bool restore) -> int {
+        for (int i = 0; i < ns; ++i) {
+            dmc_ctx *c = b->ctx[b->order[i]];
+            struct { void *p; size_t n; } r[3] = {
+                {c->T.moves, 2 * sizeof(int64_t)}, {c->T.hist, 500 * sizeof(int64_t)},
+                {c->A.sinv, size_t(c->nw) * sinv_bytes(c->ds)}};
+            for (int k = 0; k < 3; ++k) {
+                auto &v = save[3 * i + k];
+                if (!restore) v.resize(r[k].n);
+                HIPCHK(hipMemcpy(restore ? r[k].p : v.data(), restore ? v.data() : r[k].p, r[k].n,
+                                 restore ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+            }
+        }
+        return DMC_OK;
+    };
+    e = snap(false);
+    if (e) return e;
+    // the args of the current populations (no capture needed for direct launches)
+    b->args.nsys = ns;
+    for (int i = 0; i < ns; ++i) {
+        dmc_ctx *c = b->ctx[b->order[i]];
+        b->host.sys[i].S = c->ds;
+        b->host.sys[i].A = c->A;
+        b->host.sys[i].B = c->B;
+        b->host.sys[i].T = c->T;
+        b->args.sys[i] = b->host.sys[i];
+    }
+    const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
+    const int cand[4] = {1, 2, 4, 8};
+    for (int p = 0; p < 3; ++p) {
+        if (p == 1 && b->nchunk < 2) {
+            if (ms_out) for (int k = 0; k < 4; ++k) ms_out[4 * p + k] = -1.0;
+            continue;
+        }
+        double best = 1e300;
+        int bw = b->wpg[p];
+        for (int k = 0; k < 4; ++k) {
+            double t = -1.0;
+            if (batch_blocks_per_cu(b->host, ph[p], cand[k]) > 0) {
+                const int chunk = p == 2 ? -1 : (p == 1 ? 1 : 0);
+                float ms = 0.f;
+                for (int r = 0; r <= reps && !e; ++r) {  // the first launch warms up
+                    if (r == 1) e = hipEventRecord(b->e0, b->st) ? DMC_ERR_HIP : 0;
+                    if (!e) e = launch_batch_begin(b->args, b->counters, dmc_batch::kBatchCounters, b->st) ? DMC_ERR_HIP : 0;
+                    if (!e) e = launch_batch_phase(b->host, b->args, ph[p], chunk, 0, b->counters, cand[k], b->ncu, b->st) ? DMC_ERR_HIP : 0;
+                }
+                if (!e) e = hipEventRecord(b->e1, b->st) ? DMC_ERR_HIP : 0;
+                if (!e) e = hipEventSynchronize(b->e1) ? DMC_ERR_HIP : 0;
+                if (!e) e = hipEventElapsedTime(&ms, b->e0, b->e1) ? DMC_ERR_HIP : 0;
+                if (e) return fail(DMC_ERR_HIP, "tuning launch failed");
+                t = ms / reps;
+                if (t < best) {
+                    best = t;
+                    bw = cand[k];
+                }
+            }
+            if (ms_out) ms_out[4 * p + k] = t;
+        }
+        b->wpg[p] = bw;
+    }
+    e = snap(true);
+    if (e) return e;
+    b->dirty = true;
+    if (wpg_out)
+        for (int p = 0; p < 3; ++p) wpg_out[p] = b->wpg[p];
     return DMC_OK;
 }
 

@@ -37,7 +37,14 @@ int launch_branch_count(const DevSys &S, const DevState &A, const DevStep &T, do
 int launch_scan(const DevStep &T, hipStream_t st);
 int launch_map_gather(const DevSys &S, const DevState &src, const DevState &dst,
                       const DevStep &T, int nsrc, int ndst, int capacity, hipStream_t st);
+// grid < 0: the count in nt[0]
 int launch_est_reduce(const DevStep &T, int grid, double *acc, hipStream_t st);
+// branch and population control with every size on the device (graph-capturable): the
+// population enters in B with its multiplicities and leaves in A; nt[2] = population
+int launch_branch_device(const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
+                         int capacity, hipStream_t st);
+// nt[0] = nt[2] at the start of a device-driven step
+int launch_pop_begin(const DevStep &T, hipStream_t st);
 
 bool nup_supported(int nup);
 

@@ -11,6 +11,16 @@ namespace dmc {
 // ------------------------------------------------------------------------------------
 // small helpers
 
+// Ordering of the LDS traffic of one walker.  Every kernel gives a walker one 64-lane wave
+// (and a workgroup may hold several walkers, one per wave), so the lanes that exchange data
+// through LDS are always those of one wave: a wavefront-scope fence, which also keeps the
+// compiler from moving LDS accesses across it, is the synchronisation they need.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double wrapd(double x, const DevSys &S) {
     return x - S.el * round(x * S.eli);  // Fortran nint: half away from zero
 }
@@ -230,7 +240,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     // Phase A: plane-wave phases of every particle
     // q-major: ph[q * NP + p], so that lanes over particles read consecutive words
     for (int p = tid; p < NP; p += 64) phases_of(S, qm, xs + 3 * p, ph + p, NP);
-    __syncthreads();
+    wave_sync();
 
     // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij)
     double Sm[TR][TR];
@@ -316,7 +326,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
             perm[k] = p;
             pinv[p] = k;
         }
-        __syncthreads();
+        wave_sync();
         const double piv = mcol[p];
         singular |= (piv == 0.0);
         const double rp = 1.0 / piv;
@@ -344,7 +354,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
             }
             if (isp) used |= 1u << a;
         }
-        __syncthreads();
+        wave_sync();
     }
 
     // permutation parity: n - #cycles
@@ -369,7 +379,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
 #pragma unroll
         for (int b = 0; b < TR; ++b)
             if (rv[a] && cv[b]) sinv[pinv[ri[a]] * SP + perm[cj[b]]] = Sm[a][b];
-    __syncthreads();
+    wave_sync();
     return DetInfo{logdet, sgn, singular};
 }
 
@@ -482,7 +492,7 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
         }
         lap += cq[4] * ls;
     }
-    __syncthreads();  // done with ph and sinv: reuse the work area for partial sums
+    wave_sync();  // done with ph and sinv: reuse the work area for partial sums
 
     // Phase G: reduce per-particle partial sums across the lane grid
     double *pu = sm + L.work;            // [6][NUP][8]  up particles, over c
@@ -504,7 +514,7 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     double du2 = 0.0, dudp = 0.0;
     for (int p = tid; p < NP; p += 64) {
         const double *src = p < NUP ? pu : pd;
@@ -537,7 +547,7 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
         sm[L.scal + 3] = double(sgn);
         sm[L.scal + 4] = logdet;
     }
-    __syncthreads();
+    wave_sync();
 }
 
 // copy the inverse left in LDS by build_inverse to global memory, dense [j][i], as FP64
@@ -633,9 +643,11 @@ __device__ void finish_walker(const DevSys &S, double *sm, const Layout<NUP> &L,
         xn[t] = sm[L.xs + t];
         gn[t] = sm[L.dp + t];
     }
-    double *est = T.est + size_t(w) * 42;
+    // the 42 estimator values are assembled in LDS (the work area is free by now) and
+    // stored once, so that no two lanes store to one global address
+    double *est = sm + L.work;
     if (tid < 42) est[tid] = 0.0;
-    __syncthreads();
+    wave_sync();
     if (tid == 0) {
         const double psil2 = sm[L.scal], d2psi2 = sm[L.scal + 1], v2 = sm[L.scal + 2];
         const int is2 = int(sm[L.scal + 3]);
@@ -689,6 +701,8 @@ __device__ void finish_walker(const DevSys &S, double *sm, const Layout<NUP> &L,
         B.wt0[w] = wt0;
         B.weight[w] = weight;
     }
+    wave_sync();
+    if (tid < 42) T.est[size_t(w) * 42 + tid] = est[tid];
 }
 
 }  // namespace dmc

@@ -63,6 +63,13 @@ SIGNATURES = {
     "dmc_kernel_timing": [_vp, C.c_int32],
     "dmc_kernel_times": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32],
     "dmc_sync": [_vp],
+    "dmc_batch_create": [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)],
+    "dmc_batch_destroy": [_vp],
+    "dmc_batch_step": [_vp, C.c_int32],
+    "dmc_batch_run": [_vp, C.c_int32],
+    "dmc_batch_sync": [_vp, _i32],
+    "dmc_batch_set_wpg": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "dmc_batch_tune": [_vp, C.c_int32, _i32, _f64],
 }
 
 _lib = None

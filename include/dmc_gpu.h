@@ -135,6 +135,28 @@ int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset);
 /* Synchronise the context's stream. */
 int dmc_sync(dmc_ctx *c);
 
+/* Batches: up to 4 contexts (different particle numbers, each with its own tables and
+ * population) stepped together.  The reference runs one system per job; a batch has no
+ * counterpart there.  Each batch step is one captured graph: one persistent launch per phase
+ * (moves, FP64 refresh, end of sweep) covers the walkers of every system, largest particle
+ * number first, followed by each system's estimator sums, branch and population control
+ * with every size kept on the device.  Members must use single-particle moves and one
+ * precision.  Steps are asynchronous: dmc_batch_sync waits, stores each member's
+ * population (nt[i] in the caller's order) and reports overflow or a dead population; after
+ * it the members can be read or stepped on their own. */
+typedef struct dmc_batch dmc_batch;
+int dmc_batch_create(dmc_ctx *const *ctxs, int32_t n, dmc_batch **out);
+int dmc_batch_destroy(dmc_batch *b);
+int dmc_batch_step(dmc_batch *b, int32_t opcomp);
+int dmc_batch_run(dmc_batch *b, int32_t nsteps);  /* opcomp on the last step */
+int dmc_batch_sync(dmc_batch *b, int32_t *nt);
+/* walkers per workgroup (1, 2, 4 or 8; one walker per 64-lane wave) of the three phases */
+int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end);
+/* Autotuner: times reps launches of every phase at 1, 2, 4 and 8 walkers per workgroup on
+ * the resident populations (state left unchanged) and keeps the fastest.  wpg_out[3] = the
+ * choice per phase; ms_out[3][4] = ms per launch (-1: does not fit or phase unused). */
+int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out);
+
 #ifdef __cplusplus
 }
 #endif
