@@ -253,10 +253,11 @@ class Batch:
         L.check(self.lib.dmc_batch_set_wpg(self.h, int(moves), int(refresh), int(end)))
 
     def tune(self, reps=3):
-        """Walkers per workgroup of every phase from timed launches; returns the choice and
-        the ms per launch [phase][wpg = 1, 2, 4, 8] (-1: not run)."""
+        """Launch setting of every phase from timed launches; returns the choice (walkers
+        per workgroup, 0 = one launch per system) and the ms per launch
+        [phase][setting = 1, 2, 4, 8, 0] (-1: not run)."""
         w = np.zeros(3, np.int32)
-        ms = np.zeros(12)
+        ms = np.zeros(15)
         L.check(self.lib.dmc_batch_tune(self.h, int(reps), w, ms))
-        return dict(zip(("moves", "refresh", "end"), (int(v) for v in w))), ms.reshape(3, 4)
+        return dict(zip(("moves", "refresh", "end"), (int(v) for v in w))), ms.reshape(3, 5)
 

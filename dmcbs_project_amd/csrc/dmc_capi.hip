@@ -709,6 +709,35 @@ void batch_drop_graphs(dmc_batch *b) {
     b->dirty = true;
 }
 
+// One phase of a batch step.  setting = walkers per workgroup of the one persistent launch
+// over all systems, or 0: one launch of the single-system kernel per system (grid = its
+// capacity; workgroups past the device population return at once).
+int batch_phase(dmc_batch *b, BatchPhase ph, int chunk, int opcomp, int *counter, int setting) {
+    if (setting > 0)
+        return launch_batch_phase(b->host, b->args, ph, chunk, opcomp, counter, setting, b->ncu,
+                                  b->st);
+    for (size_t i = 0; i < b->order.size(); ++i) {
+        dmc_ctx *c = b->ctx[b->order[i]];
+        const BatchSys &q = b->host.sys[i];
+        int e = 0;
+        if (ph == BatchPhase::End) {
+            e = launch_sm_end(q.nup, c->ds, c->A, c->B, c->T, opcomp, c->capacity, b->st);
+        } else {
+            const int m0 = chunk * q.k;
+            if (m0 >= q.npart) continue;
+            const int m1 = m0 + q.k < q.npart ? m0 + q.k : q.npart;
+            if (ph == BatchPhase::Refresh)
+                e = launch_sm_refresh(q.nup, c->ds, c->B.x, c->A.sinv, c->T, c->capacity, b->st);
+            else
+                e = launch_sm_moves(q.nup, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
+                                    m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1, c->T,
+                                    c->capacity, b->st);
+        }
+        if (e) return e;
+    }
+    return hipSuccess;
+}
+
 // the launches of one batch step on b->st (captured, or run directly by the tuner)
 int batch_enqueue(dmc_batch *b, int opcomp) {
     const int ns = int(b->order.size());
@@ -716,13 +745,13 @@ int batch_enqueue(dmc_batch *b, int opcomp) {
     int slot = 0;
     for (int c = 0; c < b->nchunk; ++c) {
         if (c > 0)
-            LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::Refresh, c, 0,
-                                         b->counters + kMaxBatch * slot++, b->wpg[1], b->ncu, b->st));
-        LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::Moves, c, 0,
-                                     b->counters + kMaxBatch * slot++, b->wpg[0], b->ncu, b->st));
+            LAUNCHCHK(batch_phase(b, BatchPhase::Refresh, c, 0, b->counters + kMaxBatch * slot++,
+                                  b->wpg[1]));
+        LAUNCHCHK(batch_phase(b, BatchPhase::Moves, c, 0, b->counters + kMaxBatch * slot++,
+                              b->wpg[0]));
     }
-    LAUNCHCHK(launch_batch_phase(b->host, b->args, BatchPhase::End, -1, opcomp,
-                                 b->counters + kMaxBatch * slot++, b->wpg[2], b->ncu, b->st));
+    LAUNCHCHK(batch_phase(b, BatchPhase::End, -1, opcomp, b->counters + kMaxBatch * slot++,
+                          b->wpg[2]));
     for (int i = 0; i < ns; ++i) {
         dmc_ctx *c = b->ctx[b->order[i]];
         LAUNCHCHK(launch_est_reduce(c->T, -1, c->est_acc, b->st));
@@ -857,9 +886,9 @@ int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end)
     const int32_t w[3] = {moves, refresh, end};
     const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
     for (int i = 0; i < 3; ++i) {
-        if (w[i] != 1 && w[i] != 2 && w[i] != 4 && w[i] != 8)
-            return fail(DMC_ERR_ARG, "walkers per workgroup must be 1, 2, 4 or 8");
-        if (batch_blocks_per_cu(b->host, ph[i], w[i]) <= 0)
+        if (w[i] != 0 && w[i] != 1 && w[i] != 2 && w[i] != 4 && w[i] != 8)
+            return fail(DMC_ERR_ARG, "walkers per workgroup must be 1, 2, 4 or 8 (0: per system)");
+        if (w[i] > 0 && batch_blocks_per_cu(b->host, ph[i], w[i]) <= 0)
             return fail(DMC_ERR_ARG, "walkers per workgroup " + std::to_string(w[i]) +
                                          " does not fit the LDS or registers of a CU");
     }
@@ -960,23 +989,24 @@ int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out)
         b->args.sys[i] = b->host.sys[i];
     }
     const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
-    const int cand[4] = {1, 2, 4, 8};
+    constexpr int kCand = 5;
+    const int cand[kCand] = {1, 2, 4, 8, 0};
     for (int p = 0; p < 3; ++p) {
         if (p == 1 && b->nchunk < 2) {
-            if (ms_out) for (int k = 0; k < 4; ++k) ms_out[4 * p + k] = -1.0;
+            if (ms_out) for (int k = 0; k < kCand; ++k) ms_out[kCand * p + k] = -1.0;
             continue;
         }
         double best = 1e300;
         int bw = b->wpg[p];
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kCand; ++k) {
             double t = -1.0;
-            if (batch_blocks_per_cu(b->host, ph[p], cand[k]) > 0) {
+            if (cand[k] == 0 || batch_blocks_per_cu(b->host, ph[p], cand[k]) > 0) {
                 const int chunk = p == 2 ? -1 : (p == 1 ? 1 : 0);
                 float ms = 0.f;
                 for (int r = 0; r <= reps && !e; ++r) {  // the first launch warms up
                     if (r == 1) e = hipEventRecord(b->e0, b->st) ? DMC_ERR_HIP : 0;
                     if (!e) e = launch_batch_begin(b->args, b->counters, dmc_batch::kBatchCounters, b->st) ? DMC_ERR_HIP : 0;
-                    if (!e) e = launch_batch_phase(b->host, b->args, ph[p], chunk, 0, b->counters, cand[k], b->ncu, b->st) ? DMC_ERR_HIP : 0;
+                    if (!e) e = batch_phase(b, ph[p], chunk, 0, b->counters, cand[k]) ? DMC_ERR_HIP : 0;
                 }
                 if (!e) e = hipEventRecord(b->e1, b->st) ? DMC_ERR_HIP : 0;
                 if (!e) e = hipEventSynchronize(b->e1) ? DMC_ERR_HIP : 0;
@@ -988,7 +1018,7 @@ int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out)
                     bw = cand[k];
                 }
             }
-            if (ms_out) ms_out[4 * p + k] = t;
+            if (ms_out) ms_out[kCand * p + k] = t;
         }
         b->wpg[p] = bw;
     }

@@ -150,11 +150,13 @@ int dmc_batch_destroy(dmc_batch *b);
 int dmc_batch_step(dmc_batch *b, int32_t opcomp);
 int dmc_batch_run(dmc_batch *b, int32_t nsteps);  /* opcomp on the last step */
 int dmc_batch_sync(dmc_batch *b, int32_t *nt);
-/* walkers per workgroup (1, 2, 4 or 8; one walker per 64-lane wave) of the three phases */
+/* Launch setting of the three phases: walkers per workgroup (1, 2, 4 or 8; one walker per
+ * 64-lane wave) of the one persistent launch over all systems, or 0 for one launch of the
+ * single-system kernel per system (all inside the step graph). */
 int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end);
-/* Autotuner: times reps launches of every phase at 1, 2, 4 and 8 walkers per workgroup on
- * the resident populations (state left unchanged) and keeps the fastest.  wpg_out[3] = the
- * choice per phase; ms_out[3][4] = ms per launch (-1: does not fit or phase unused). */
+/* Autotuner: times reps launches of every phase at the settings 1, 2, 4, 8 and 0 on the
+ * resident populations (state left unchanged) and keeps the fastest.  wpg_out[3] = the
+ * choice per phase; ms_out[3][5] = ms per launch (-1: does not fit or phase unused). */
 int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out);
 
 #ifdef __cplusplus
