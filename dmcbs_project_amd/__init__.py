@@ -4,8 +4,9 @@ The device library libdmc_gpu.so (HIP kernels for gfx950 behind the C-ABI of
 include/dmc_gpu.h) does the work; this package is a thin host-side binding whose classes
 mirror the reference's Fortran module interfaces (step, stack, wavefunction).
 """
-from .lib import DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMCError, load
+from .lib import (DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC, DMCError,
+                  load)
 from .api import Params, Engine, Batch
 
 __all__ = ["Params", "Engine", "Batch", "DMCError", "load", "DMC_FP64", "DMC_MIXED", "DMC_ALLPARTICLE",
-           "DMC_SINGLEPARTICLE"]
+           "DMC_SINGLEPARTICLE", "DMC_VMC"]

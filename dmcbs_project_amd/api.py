@@ -131,8 +131,46 @@ class Engine:
                                      de.ctypes.data if de is not None else None))
         return (dl, de) if with_elocal else dl
 
+    # --- optimizer.f90 -------------------------------------------------------------------
+    def calder(self):
+        """calder (optimizer.f90:90-112) for every resident walker, on the device."""
+        L.check(self.lib.dmc_opt_accumulate(self.h))
+
+    def updatepar(self, delta, eps, optbeta=False):
+        """updatepar (optimizer.f90:114-169) from the accumulated sums: f = -2<O E> +
+        2<O><E>, S = <O O^T> - <O><O>^T + eps I, theta += delta S^-1 f; the new parameters
+        are set (setdetparam) and the sums cleared.  Returns (<E>, new parameters)."""
+        P = self.params.norb + 2
+        sums = np.zeros(3 + 2 * P + P * P)
+        L.check(self.lib.dmc_opt_sums(self.h, sums, 1))
+        n = sums[0]
+        if n <= 0:
+            raise L.DMCError(1, "no calder samples")
+        k = P if optbeta else self.params.norb
+        ovec = sums[3:3 + P][:k] / n
+        ovecw = sums[3 + P:3 + 2 * P][:k] / n
+        pmat = sums[3 + 2 * P:].reshape(P, P)[:k, :k] / n
+        esum = sums[1] / n
+        fvec = -2.0 * ovecw + 2.0 * ovec * esum
+        pmat = pmat - np.outer(ovec, ovec) + eps * np.eye(k)
+        dpar = np.linalg.solve(pmat, fvec)
+        par = np.concatenate([self.vorb(), [self.params.bbeta, self.params.dbeta]])[:k]
+        par = par + delta * dpar
+        vorb = par[:self.params.norb].copy()
+        if optbeta:
+            self.setdetparam(vorb, par[-2], par[-1])
+        else:
+            self.setdetparam(vorb)
+        return float(esum), par
+
+    def vorb(self):
+        """The current alpha_1..alpha_norb (getdetparam, wavefunction.f90:246-258)."""
+        vo = self.params.vorb if self.params.vorb is not None else VORB10[: self.params.norb]
+        return np.array(vo[: self.params.norb], np.float64)
+
     def setdetparam(self, vorb, bbeta=None, dbeta=None):
         vorb = np.ascontiguousarray(vorb, np.float64)
+        self.params.vorb = tuple(float(v) for v in vorb)
         L.check(self.lib.dmc_set_params(self.h, vorb,
                                         self.params.bbeta if bbeta is None else float(bbeta),
                                         self.params.dbeta if dbeta is None else float(dbeta)))

@@ -86,6 +86,122 @@ __global__ __launch_bounds__(64) void k_step(DevSys S, DevState A, DevState B, D
     finish_walker<NUP>(S, sm, L, A, B, T, w, tid, s1, opcomp);
 }
 
+// Gaussian pair m of gaussian(M) for the walker whose RNG state is s0 (ran.f90:85-102): the
+// values of coordinates 2m and 2m+1 (the last value of an odd count uses cos)
+__device__ __forceinline__ void gauss_pair(const DevSys &S, uint64_t s0, int m, int M, double &g0,
+                                           double &g1) {
+    const int t0 = 2 * m, t1 = 2 * m + 1;
+    const uint64_t a0 = (S.ja1[t0] * s0 + S.jc1[t0]) & kMask48;
+    const uint64_t a1 = (S.ja1[t1] * s0 + S.jc1[t1]) & kMask48;
+    const double u1 = double(a0 | 1ull) * 0x1p-48, u2 = double(a1 | 1ull) * 0x1p-48;
+    double sn, cs;
+    sincos(2.0 * kPi * u1, &sn, &cs);
+    const double rad = sqrt(-2.0 * log(u2));
+    g0 = (t1 == M ? cs : sn) * rad;
+    g1 = cs * rad;
+}
+
+// The variational step of the reference (step1 with idmc = 0, step.f90:56-76,78-122): the
+// same all-particle drift-diffusion proposal as k_step, then the Metropolis test with the
+// ratio of the drift-diffusion Green's functions, one more ran1 draw, and the six VMC
+// estimators (total, kinetic, potential, external potential, Jackson-Feenberg energy,
+// acceptance) of the state kept.  Walker w of A goes to slot w of B, accepted or not; there
+// is no branching.
+template <int NUP>
+__global__ __launch_bounds__(64) void k_vmc(DevSys S, DevState A, DevState B, DevStep T) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= T.nt[0]) return;
+    constexpr int M = 6 * NUP;
+    constexpr int ND = 2 * ((M + 1) / 2);
+    load_q<NUP>(S, sm, L, tid);
+    const uint64_t s0 = A.irn[w];
+    const double sig = S.sigma, dfac = S.sig2 * S.sqrtm;
+    const double *xo = A.x + size_t(w) * M, *go = A.dpsi + size_t(w) * M;
+    for (int m = tid; 2 * m < M; m += 64) {
+        double g[2];
+        gauss_pair(S, s0, m, M, g[0], g[1]);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int t = 2 * m + e;
+            if (t < M) {
+                double drift = go[t] * dfac;
+                if (S.cutdrift) drift = fmin(S.dcut, fmax(-S.dcut, drift));
+                sm[L.xs + t] = wrapd(wrapd(xo[t], S) + (drift + sig * g[e]), S);
+            }
+        }
+    }
+    wave_sync();
+    hpsi_block<NUP>(S, sm, L, tid);
+    // Green's-function term sum_i (do_i + dn_i).(do_i - dn_i - 2 dx_i) / (2 sigma^2), and the
+    // squared gradients of both states for the Jackson-Feenberg energy
+    double gterm = 0.0, dot_o = 0.0, dot_n = 0.0;
+    for (int m = tid; 2 * m < M; m += 64) {
+        double g[2];
+        gauss_pair(S, s0, m, M, g[0], g[1]);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int t = 2 * m + e;
+            if (t < M) {
+                double dro = go[t] * dfac, drn = sm[L.dp + t] * dfac;
+                if (S.cutdrift) {
+                    dro = fmin(S.dcut, fmax(-S.dcut, dro));
+                    drn = fmin(S.dcut, fmax(-S.dcut, drn));
+                }
+                const double dx = dro + sig * g[e];
+                gterm += (dro + drn) * (dro - drn - 2.0 * dx);
+                dot_o += go[t] * go[t];
+                dot_n += sm[L.dp + t] * sm[L.dp + t];
+            }
+        }
+    }
+    gterm = wave_sum(gterm);
+    dot_o = wave_sum(dot_o) * S.invm;
+    dot_n = wave_sum(dot_n) * S.invm;
+    const double psil2 = sm[L.scal], d2psi2 = sm[L.scal + 1], v2 = sm[L.scal + 2];
+    const int is2 = int(sm[L.scal + 3]);
+    const double prob = exp(fmin(0.0, 2.0 * (psil2 - A.psil[w]) + gterm / (2.0 * S.sig2)));
+    uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;  // after gaussian(M)
+    s1 = (s1 * kM1 + kA1) & kMask48;                                 // randn(1)
+    const bool acc = double(s1 | 1ull) * 0x1p-48 < prob;
+    double *xn = B.x + size_t(w) * M, *gn = B.dpsi + size_t(w) * M;
+    for (int t = tid; t < M; t += 64) {
+        xn[t] = acc ? sm[L.xs + t] : wrapd(xo[t], S);
+        gn[t] = acc ? sm[L.dp + t] : go[t];
+    }
+    // the six estimators of the kept state (addval: val*wt, val^2*wt, wt), in LDS first
+    double *est = sm + L.work;
+    if (tid < 42) est[tid] = 0.0;
+    wave_sync();
+    if (tid == 0) {
+        const double d2 = acc ? d2psi2 : A.d2psi[w], v = acc ? v2 : A.v[w];
+        const double dot = acc ? dot_n : dot_o, wgt = A.weight[w], nf = S.nfac;
+        const double en = -S.hbar * d2 + v;
+        const double ejf = -0.5 * S.hbar * (d2 - dot) + v;
+        const double val[6] = {en * nf, (en - v) * nf, v * nf, 0.0, ejf * nf, acc ? 1.0 : 0.0};
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const double wt = e == 5 ? 1.0 : wgt;
+            est[3 * e] = val[e] * wt;
+            est[3 * e + 1] = val[e] * val[e] * wt;
+            est[3 * e + 2] = wt;
+        }
+        B.irn[w] = s1;
+        B.weight[w] = wgt;
+        B.psil[w] = acc ? psil2 : A.psil[w];
+        B.d2psi[w] = d2;
+        B.v[w] = v;
+        B.is[w] = acc ? is2 : A.is[w];
+        B.wt0[w] = acc ? double(is2) : A.wt0[w];
+        T.nacc[w] = acc ? 1 : 0;
+        if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[0]), 1ull);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[1]), 1ull);
+    }
+    wave_sync();
+    if (tid < 42) T.est[size_t(w) * 42 + tid] = est[tid];
+}
+
 // single-particle ratio psi(x')/psi(x) (move1, wavefunction.f90:393-404): determinant
 // ratio from the inverse (bcsone, psislaterbcs.f90:296-376) times the Jastrow ratio
 template <int NUP>
@@ -163,6 +279,14 @@ int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
     if (grid <= 0) return hipSuccess;
     DMC_DISPATCH(nup, hipLaunchKernelGGL(k_step<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq), st,
                                          S, A, B, T, opcomp));
+    return hipGetLastError();
+}
+
+int launch_vmc(int nup, const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
+               int grid, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_vmc<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq), st,
+                                         S, A, B, T));
     return hipGetLastError();
 }
 

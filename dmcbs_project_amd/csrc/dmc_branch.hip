@@ -124,6 +124,46 @@ __global__ __launch_bounds__(256) void k_est_reduce(DevStep T, int nw, double *a
     if (tid == 0) acc[k] += part[0];
 }
 
+// calder (optimizer.f90:90-112) over walkers 0..nw-1: acc[k] += sum_w term_k(w) in a fixed
+// order, one block per term: k = 0 count, 1 E, 2 E^2, 3.. O_p, 3+P.. O_p E, 3+2P.. O_p O_q,
+// with E = E_L / (N eunit) of the stored state and O = d log psi / d theta [nw][P]
+__global__ __launch_bounds__(256) void k_opt_accumulate(DevSys S, DevState A, const double *O,
+                                                        int nw, int P, double *acc) {
+    __shared__ double part[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    double s = 0.0;
+    for (int w = tid; w < nw; w += 256) {
+        const double e = (-S.hbar * A.d2psi[w] + A.v[w]) * S.nfac;
+        const double *o = O + size_t(w) * P;
+        double t;
+        if (k == 0) t = 1.0;
+        else if (k == 1) t = e;
+        else if (k == 2) t = e * e;
+        else if (k < 3 + P) t = o[k - 3];
+        else if (k < 3 + 2 * P) t = o[k - 3 - P] * e;
+        else {
+            const int pq = k - 3 - 2 * P;
+            t = o[pq / P] * o[pq % P];
+        }
+        s += t;
+    }
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) part[tid] += part[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) acc[k] += part[0];
+}
+
+int launch_opt_accumulate(const DevSys &S, const DevState &A, const double *O, int nw, int P,
+                          double *acc, hipStream_t st) {
+    if (nw <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_opt_accumulate, dim3(3 + 2 * P + P * P), dim3(256), 0, st, S, A, O, nw,
+                       P, acc);
+    return hipGetLastError();
+}
+
 // device-driven population bookkeeping after a branch pass: resident population
 // nt[2] = min(total, capacity), overflow flag nt[3]; then (decide != 0) the population-
 // control test of dmcbcs.f90:444-445, setting nt[4] and, when it fires, nt[0] = nt[2]

@@ -65,6 +65,8 @@ struct dmc_ctx {
     DevState A{}, B{};
     DevStep T{};
     double *est_acc = nullptr;
+    double *opt_acc = nullptr;  // optimiser sums, kOptSums
+    static constexpr int kOptSums = 3 + 2 * (DMC_MAXORB + 2) + (DMC_MAXORB + 2) * (DMC_MAXORB + 2);
     int *h_nt = nullptr;  // pinned
     bool timing = false;
     double time_ms = 0.0;
@@ -256,17 +258,18 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         delete c;
         return fail(DMC_ERR_ARG, "precision must be DMC_FP64 or DMC_MIXED");
     }
-    if (p->movetype != DMC_ALLPARTICLE && p->movetype != DMC_SINGLEPARTICLE) {
+    if (p->movetype != DMC_ALLPARTICLE && p->movetype != DMC_SINGLEPARTICLE &&
+        p->movetype != DMC_VMC) {
         delete c;
-        return fail(DMC_ERR_ARG, "movetype must be DMC_ALLPARTICLE or DMC_SINGLEPARTICLE");
+        return fail(DMC_ERR_ARG, "movetype must be DMC_ALLPARTICLE, DMC_SINGLEPARTICLE or DMC_VMC");
     }
     if (p->krefresh < 0) {
         delete c;
         return fail(DMC_ERR_ARG, "krefresh must be >= 0");
     }
-    if (p->movetype == DMC_ALLPARTICLE && p->precision != DMC_FP64) {
+    if (p->movetype != DMC_SINGLEPARTICLE && p->precision != DMC_FP64) {
         delete c;
-        return fail(DMC_ERR_ARG, "all-particle moves run in FP64 only");
+        return fail(DMC_ERR_ARG, "all-particle and VMC moves run in FP64 only");
     }
     c->capacity = p->capacity > 0 ? p->capacity : 4 * (p->nwalk > 0 ? p->nwalk : 1);
     if (hipStreamCreate(&c->st) != hipSuccess) {
@@ -289,7 +292,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         c->alloc(&c->T.copy_irn, c->capacity) || c->alloc(&c->T.est, size_t(c->capacity) * 42) ||
         c->alloc(&c->T.nt, 8) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
         c->alloc(&c->T.moves, 2) ||
-        c->alloc(&c->est_acc, 42)) {
+        c->alloc(&c->est_acc, 42) || c->alloc(&c->opt_acc, dmc_ctx::kOptSums)) {
         std::string e = g_err;
         dmc_destroy(c);
         return fail(DMC_ERR_HIP, e);
@@ -322,6 +325,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         hipMemcpy(ja2, a2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(jc2, c2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->est_acc, 0, 42 * sizeof(double)) != hipSuccess ||
+        hipMemset(c->opt_acc, 0, dmc_ctx::kOptSums * sizeof(double)) != hipSuccess ||
         hipMemset(c->T.hist, 0, 500 * sizeof(int64_t)) != hipSuccess ||
         hipMemset(c->T.moves, 0, 2 * sizeof(int64_t)) != hipSuccess)
         e = DMC_ERR_HIP;
@@ -532,6 +536,11 @@ int dmc_propagate(dmc_ctx *c, int32_t opcomp) {
             if (ev) HIPCHK(hipEventRecord(c->evs[2 * nev++ + 1], c->st));
         }
         LAUNCHCHK(launch_sm_end(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
+    } else if (c->sys.p.movetype == DMC_VMC) {
+        LAUNCHCHK(launch_vmc(c->sys.n, c->ds, c->A, c->B, c->T, n, c->st));
+        LAUNCHCHK(launch_est_reduce(c->T, n, c->est_acc, c->st));
+        std::swap(c->A, c->B);
+        return DMC_OK;
     } else {
         if (c->timing) HIPCHK(hipEventRecord(c->evs[0], c->st));
         LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
@@ -563,6 +572,8 @@ int dmc_branch(dmc_ctx *c, double factor, int32_t *nt) {
         }
         c->nev_pending = 0;
     } else {
+        if (c->sys.p.movetype == DMC_VMC && factor == 1.0)
+            return fail(DMC_ERR_ARG, "VMC populations are not branched");
         // population control: int(weight * factor + u) copies of every walker
         if (n <= 0) return fail(DMC_ERR_ARG, "population died out");
         if (!(factor > 0.0)) return fail(DMC_ERR_ARG, "branch factor must be positive");
@@ -581,6 +592,10 @@ int dmc_branch(dmc_ctx *c, double factor, int32_t *nt) {
 int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt) {
     int e = dmc_propagate(c, opcomp);
     if (e) return e;
+    if (c->sys.p.movetype == DMC_VMC) {
+        if (nt) nt[0] = nt[1] = c->nw;
+        return DMC_OK;
+    }
     int total = 0, final_n = 0;
     e = dmc_branch(c, 1.0, &total);
     if (e) return e;
@@ -653,6 +668,29 @@ int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset) {
 
 int dmc_sync(dmc_ctx *c) {
     if (!c) return fail(DMC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_opt_accumulate(dmc_ctx *c) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
+    const int nw = c->nw, P = c->sys.p.norb + 2;
+    if (nw <= 0) return DMC_OK;
+    int e = c->ensure_scratch(sizeof(double) * size_t(nw) * P + 256);
+    if (e) return e;
+    double *O = static_cast<double *>(c->scratch);
+    LAUNCHCHK(launch_adjoint(c->sys.n, c->ds, c->A.x, nw, c->sys.p.norb, O, nullptr, c->st));
+    LAUNCHCHK(launch_opt_accumulate(c->ds, c->A, O, nw, P, c->opt_acc, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_opt_sums(dmc_ctx *c, double *sums, int32_t reset) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    const int P = c->sys.p.norb + 2, n = 3 + 2 * P + P * P;
+    if (sums) HIPCHK(hipMemcpyAsync(sums, c->opt_acc, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    if (reset) HIPCHK(hipMemsetAsync(c->opt_acc, 0, sizeof(double) * dmc_ctx::kOptSums, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return DMC_OK;
 }

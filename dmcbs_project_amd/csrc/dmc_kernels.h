@@ -16,6 +16,9 @@ int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const
                  int nw, double *ratio, hipStream_t st);
 int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
                 const DevStep &T, int opcomp, int grid, hipStream_t st);
+// variational (Metropolis) step of state A into state B, no branching
+int launch_vmc(int nup, const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
+               int grid, hipStream_t st);
 // single-particle moves m0..m1-1 of every walker, from positions/RNG xin/irn_in and the
 // stored inverse sinv, to xout/irn_out (move kernel)
 int launch_sm_moves(int nup, const DevSys &S, const double *xin, double *xout,
@@ -43,6 +46,9 @@ int launch_est_reduce(const DevStep &T, int grid, double *acc, hipStream_t st);
 // population enters in B with its multiplicities and leaves in A; nt[2] = population
 int launch_branch_device(const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
                          int capacity, hipStream_t st);
+// optimiser sums (calder) of the walkers of A with their parameter derivatives O [nw][P]
+int launch_opt_accumulate(const DevSys &S, const DevState &A, const double *O, int nw, int P,
+                          double *acc, hipStream_t st);
 // nt[0] = nt[2] at the start of a device-driven step
 int launch_pop_begin(const DevStep &T, hipStream_t st);
 

@@ -13,7 +13,7 @@ LIBPATH = os.path.join(HERE, "libdmc_gpu.so")
 MAXORB = 16
 
 DMC_FP64, DMC_MIXED = 0, 1
-DMC_ALLPARTICLE, DMC_SINGLEPARTICLE = 0, 1
+DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC = 0, 1, 2
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "population overflow",
           4: "set-up failure", 5: "no HIP device"}
 
@@ -60,6 +60,8 @@ SIGNATURES = {
     "dmc_wtisto": [_vp, _i64],
     "dmc_acceptance": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "dmc_adjoint": [_vp, C.c_int32, _f64, _f64, _vp],
+    "dmc_opt_accumulate": [_vp],
+    "dmc_opt_sums": [_vp, _f64, C.c_int32],
     "dmc_kernel_timing": [_vp, C.c_int32],
     "dmc_kernel_times": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32],
     "dmc_sync": [_vp],

@@ -32,9 +32,11 @@ enum {
 
 /* precision of the propagator */
 enum { DMC_FP64 = 0, DMC_MIXED = 1 };
-/* move type: the reference's all-particle move (step.f90:56-76), or single-particle moves
- * with Sherman-Morrison inverse updates and an FP64 LU refresh every krefresh moves */
-enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1 };
+/* move type: the reference's all-particle DMC move (step.f90:56-76), single-particle DMC
+ * moves with Sherman-Morrison inverse updates and an FP64 LU refresh every krefresh moves, or
+ * the reference's variational Metropolis step (step1 with idmc = 0, step.f90:78-122: no
+ * weights, no branching; estimators 1-6 every step) */
+enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1, DMC_VMC = 2 };
 
 /* Run parameters: the reference's deck (dmcbcs.f90:43-66, psislaterbcs.f90:47-68,
  * betafun.f90:304-306, coshpot.f90:35-41) plus the device options. */
@@ -106,7 +108,8 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt);
  * branch draw of every walker; the population is not changed until dmc_branch.
  * dmc_branch replaces branch (step.f90:287-315).  Right after dmc_propagate it takes
  * factor 1 and applies the drawn multiplicities; otherwise it is a population-control pass
- * with multiplicities int(weight*factor + u).  nt = the population afterwards. */
+ * with multiplicities int(weight*factor + u).  nt = the population afterwards.  DMC_VMC
+ * populations are not branched (dmcbcs.f90:441): dmc_propagate leaves them in place. */
 int dmc_propagate(dmc_ctx *c, int32_t opcomp);
 int dmc_branch(dmc_ctx *c, double factor, int32_t *nt);
 /* Run nsteps steps (opcomp on the last), returning the final population. */
@@ -125,6 +128,15 @@ int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed);
  * betafun.f90:481-612; the reference computes no d E_L / d theta).  The inverse is
  * recomputed, not stored.  dlogpsi and delocal are [nw][norb+2]; delocal may be NULL. */
 int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double *delocal);
+
+/* Optimiser accumulation (calder, optimizer.f90:90-112) over the resident population, with
+ * E = E_L / (npart eunit) of each walker's stored state and O = d log psi / d theta for
+ * theta = (alpha_1..alpha_norb, b, d), P = norb + 2, from the adjoint kernel.  The sums
+ * accumulate across calls in a fixed order: sums = [n, sum E, sum E^2, sum O_p (P),
+ * sum O_p E (P), sum O_p O_q (P*P, row-major)]; reset = 1 zeroes them afterwards.  The
+ * update itself (updatepar, optimizer.f90:114-169: a P x P solve) is host arithmetic. */
+int dmc_opt_accumulate(dmc_ctx *c);
+int dmc_opt_sums(dmc_ctx *c, double *sums, int32_t reset);
 
 /* Timing of the dominant kernel with HIP events recorded on the stream it is launched on:
  * the move kernel (k_sm_moves, every chunk of krefresh moves) for single-particle moves, the

@@ -69,6 +69,8 @@ def lib():
         L.dmco_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
                                  C.c_int32, _f64p, _f64p, _f64p, _f64p, _i64p, _f64p,
                                  C.c_int32]
+        L.dmco_vmc_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                     _f64p, _f64p, _f64p, _f64p, _i64p, C.c_int32]
         L.dmco_branch.argtypes = [C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _i64p,
                                   C.c_double, C.c_int32, _f64p, _f64p, _f64p, _i64p]
         L.dmco_branch.restype = C.c_int
@@ -219,6 +221,18 @@ class System:
                           do.reshape(-1), so.reshape(-1), ro, wt, nthreads)
         return xo, do, so, ro, wt, est
 
+    def vmc_step1(self, x, dpsi, sc, irn, est=None, nthreads=0):
+        """step1 with idmc = 0 (Metropolis); returns x, dpsi, sc, irn, est."""
+        nw = len(x)
+        est = np.zeros(42) if est is None else est
+        xo, do = np.zeros_like(x), np.zeros_like(dpsi)
+        so, ro = np.zeros_like(sc), np.zeros_like(irn)
+        self.L.dmco_vmc_step1(self.h, nw, np.ascontiguousarray(x).reshape(-1),
+                              np.ascontiguousarray(dpsi).reshape(-1),
+                              np.ascontiguousarray(sc).reshape(-1), np.ascontiguousarray(irn),
+                              est, xo.reshape(-1), do.reshape(-1), so.reshape(-1), ro, nthreads)
+        return xo, do, so, ro, est
+
     def dmc_step(self, x, dpsi, sc, irn, opcomp=False, est=None, cap=None, nthreads=0):
         nw = len(x)
         cap = cap or max(4 * nw, 64)
@@ -249,6 +263,35 @@ class System:
                              int(krefresh), xo.reshape(-1), do.reshape(-1), so.reshape(-1), ro,
                              wt, nacc, nthreads)
         return xo, do, so, ro, wt, nacc
+
+
+def calder_updatepar(system, configs, delta, eps, optbeta=False):
+    """One optimisation update of the determinant parameters (optimizer.f90:90-169): calder
+    on every configuration (E_L per particle in units of eunit, d log psi from getderbcs),
+    then updatepar: f = -2<O E> + 2<O><E>, S = <O O> - <O><O> + eps, new = old + delta S^-1 f.
+    Returns (<E>, new parameters)."""
+    p = system.params
+    hbar, npart = p["hbar"], system.npart
+    par = np.concatenate([system.vorb, [p["bbeta"], p["dbeta"]]]) if optbeta else system.vorb.copy()
+    npar = len(par)
+    ovec, ovecw = np.zeros(npar), np.zeros(npar)
+    pmat = np.zeros((npar, npar))
+    esum = esum2 = 0.0
+    for x in configs:
+        o = system.hpsi(x)
+        ene = (-hbar * o["d2psi"] + o["v"]) / system.eunit / npart
+        d = system.getder(x, optbeta=optbeta)
+        ovec += d
+        esum += ene
+        esum2 += ene * ene
+        ovecw += d * ene
+        pmat += np.outer(d, d)
+    n = len(configs)
+    ovec, ovecw, pmat, esum = ovec / n, ovecw / n, pmat / n, esum / n
+    fvec = -2.0 * ovecw + 2.0 * ovec * esum
+    pmat = pmat - np.outer(ovec, ovec) + eps * np.eye(npar)
+    dpar = np.linalg.inv(pmat) @ fvec
+    return esum, par + delta * dpar
 
 
 def ran1(s):

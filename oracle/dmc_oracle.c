@@ -906,6 +906,87 @@ void dmco_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dp
     free(pe);
 }
 
+/* step.f90:56-76,78-122 step1 with idmc = 0 (variational): the same proposal as step1,
+ * then the Metropolis test with the drift-diffusion Green's functions and one more ran1
+ * draw; the walker kept is the new one (wt0 = is) or the old one, with the RNG state after
+ * that draw.  Estimators 1-6 of dmcbcs.f90:180-186 every step. */
+static void vmc_one(const dmco_ctx *c, const double *x, const double *dpsi, const double *sc,
+                    int64_t irn, double *est, double *xo, double *dpsio, double *sco,
+                    int64_t *irno) {
+    int np = c->npart;
+    double hbar = c->p.hbar, el = c->el;
+    double *g = malloc(sizeof(double) * 3 * np), *x1 = malloc(sizeof(double) * 3 * np);
+    double *dro = malloc(sizeof(double) * 3 * np), *dx = malloc(sizeof(double) * 3 * np);
+    double *xn = malloc(sizeof(double) * 3 * np), *dn = malloc(sizeof(double) * 3 * np);
+    for (int i = 0; i < 3 * np; i++) x1[i] = wrapd(x[i], el);
+    int64_t s = irn;
+    dmco_gaussian(&s, 3 * np, g);
+    double sig = c->sigma, sm = sqrt(c->p.mass), dc = sig * c->p.dcut;
+    for (int i = 0; i < 3 * np; i++) {
+        double drift = dpsi[i] * (sig * sig) * sm;
+        if (c->p.dcut >= 0.0) drift = fmin(dc, fmax(-dc, drift));
+        dro[i] = drift;
+        dx[i] = drift + sig * g[i];
+        xn[i] = wrapd(x1[i] + dx[i], el);
+    }
+    double o[4];
+    dmco_hpsi(c, xn, o, dn);
+    double prob = 2.0 * (o[0] - sc[1]);
+    for (int i = 0; i < np; i++) {
+        double dot = 0.0;
+        for (int d = 0; d < 3; d++) {
+            int t = 3 * i + d;
+            double drn = dn[t] * (sig * sig) * sm;
+            if (c->p.dcut >= 0.0) drn = fmin(dc, fmax(-dc, drn));
+            dot += (dro[t] + drn) * (dro[t] - drn - 2.0 * dx[t]);
+        }
+        prob += dot / (2.0 * sig * sig);
+    }
+    prob = exp(fmin(0.0, prob));
+    double rn = dmco_ran1(&s);
+    int acc = rn < prob;
+    const double *xk = acc ? xn : x1, *dk = acc ? dn : dpsi;
+    double d2 = acc ? o[1] : sc[2], v = acc ? o[2] : sc[3], dot = 0.0;
+    for (int t = 0; t < 3 * np; t++) dot += dk[t] * dk[t] / c->p.mass;
+    double en = -hbar * d2 + v, ejf = -0.5 * hbar * (d2 - dot) + v, nfac = c->nfac;
+    double wgt = sc[0];
+    addval(est, 1, en * nfac, wgt);
+    addval(est, 2, (en - v) * nfac, wgt);
+    addval(est, 3, v * nfac, wgt);
+    addval(est, 4, 0.0, wgt);
+    addval(est, 5, ejf * nfac, wgt);
+    addval(est, 6, acc ? 1.0 : 0.0, 1.0);
+    memcpy(xo, xk, sizeof(double) * 3 * np);
+    memcpy(dpsio, dk, sizeof(double) * 3 * np);
+    sco[0] = wgt;
+    sco[1] = acc ? o[0] : sc[1];
+    sco[2] = d2;
+    sco[3] = v;
+    sco[4] = acc ? o[3] : sc[4];
+    sco[5] = acc ? o[3] : sc[5];
+    *irno = s;
+    free(g); free(x1); free(dro); free(dx); free(xn); free(dn);
+}
+
+void dmco_vmc_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                    const double *sc, const int64_t *irn, double *est, double *x_out,
+                    double *dpsi_out, double *sc_out, int64_t *irn_out, int32_t nthreads) {
+    int np = c->npart;
+    double *pe = calloc((size_t)nw * 42, sizeof(double));
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int w = 0; w < nw; w++) {
+        size_t o = (size_t)3 * np * w;
+        vmc_one(c, x + o, dpsi + o, sc + 6 * w, irn[w], pe + 42 * w, x_out + o, dpsi_out + o,
+                sc_out + 6 * w, irn_out + w);
+    }
+    for (int w = 0; w < nw; w++)
+        for (int k = 0; k < 42; k++) est[k] += pe[42 * w + k];
+    free(pe);
+}
+
 /* step.f90:287-315 branch */
 int dmco_branch(int32_t nw, int32_t npart, const double *x, const double *dpsi,
                 const double *sc, const int64_t *irn, double factor, int32_t cap,

@@ -85,6 +85,9 @@ void dmco_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dp
                 double *x_out, double *dpsi_out, double *sc_out, int64_t *irn_out,
                 double *wt_out, int32_t nthreads);
 /* branch with factor: returns new count (or -1 if > cap) (step.f90:287-315) */
+void dmco_vmc_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                    const double *sc, const int64_t *irn, double *est, double *x_out,
+                    double *dpsi_out, double *sc_out, int64_t *irn_out, int32_t nthreads);
 int dmco_branch(int32_t nw, int32_t npart, const double *x, const double *dpsi,
                 const double *sc, const int64_t *irn, double factor, int32_t cap,
                 double *x_out, double *dpsi_out, double *sc_out, int64_t *irn_out);

@@ -281,6 +281,9 @@ contains
    enddo
    close (22)
    call updatepar(ene)
+! the optimiser's new parameters into the wavefunction (optimizer.f90:73-88), as the next
+! optimisation iteration of dmcbcs.f90:236 does
+   call updateparams
    call getdetparam(npar,par)
    call openout
    write (21) ene
@@ -379,7 +382,8 @@ contains
       do j=1,nstep
          jj=jj+1
          call step1(istin,istout,j.eq.nstep)
-         call branch(istin,istout,1.0_r8)
+! dmcbcs.f90:441: no branching in variational runs
+         if (idmc.gt.0.or.idmc.eq.-2) call branch(istin,istout,1.0_r8)
          nt=numstack(istin)
          write (21) int(nt,i8)
          if (nt.gt.popup*nwalk.or.nt.lt.popdn*nwalk) then

@@ -5,6 +5,7 @@ Runs oracle/_ref/ref_driver (the reference's Fortran modules compiled by
 Only this script touches the reference build; the tests read the .npz files.
 
     python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py --vmc      # the variational and optimiser ones
 """
 import os
 import sys
@@ -127,6 +128,21 @@ def der_fixture(n, norb, configs, tag):
              dlogpsi=dp, deck=deck)
 
 
+def opt_fixture(n, norb, configs, tag, delta=0.7, eps=1.0e-4):
+    """calder over configurations, updatepar, then the new parameters (do_opt)."""
+    deck, p = refdeck.make_deck(npart=n, norb=norb, idmc=-1, delta=delta, eps=eps)
+    nc = len(configs)
+    data, _ = refrun.run("opt", deck, pack([nc], configs))
+    r = Reader(data)
+    ene = r.f()
+    npar = r.i()
+    par = r.f(npar)
+    assert r.done()
+    np.savez(os.path.join(HERE, f"opt_{tag}.npz"), npart=n, norb=norb, x=configs,
+             vorb=np.asarray(refdeck.VORB10[:norb]), ene=ene, par=par, delta=delta, eps=eps,
+             deck=deck)
+
+
 def read_dump(r, n):
     nw = r.i()
     irn = np.zeros(nw, np.int64)
@@ -183,7 +199,18 @@ def dmc_fixture(tag, dumps, keep_x=True, init=None, **deckargs):
     return out
 
 
+def main_vmc():
+    """Variational runs (step1 with idmc = 0) and the optimiser (calder, updatepar)."""
+    dmc_fixture("vmc_N14", [0, 1, 2, 10, 20], idmc=0, npart=14, norb=3, nwalk=128, nav=2,
+                nstep=10, dt=0.002, seed=11)
+    dmc_fixture("vmc_N66", [0, 1, 3], idmc=0, npart=66, norb=5, nwalk=24, nav=1, nstep=3,
+                dt=0.001, seed=11)
+    opt_fixture(14, 3, random_configs(14, 64, 6014), "N14")
+    opt_fixture(66, 5, random_configs(66, 16, 6066), "N66")
+
+
 def main():
+    main_vmc()
     for n, norb in SYSTEMS.items():
         setup_fixture(n, norb)
     # hpsi on uniform configurations (all three systems) and with 8 shells at N=66
@@ -211,4 +238,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main_vmc() if "--vmc" in sys.argv else main()

@@ -158,3 +158,40 @@ def test_dmc_cfg1_population_trace(oracle_mod):
     got = np.array(blocks)
     for i in (0, 1, 2, 5):  # total, kinetic, potential, growth energy
         np.testing.assert_allclose(got[:, i], ref[:, i], rtol=1e-8)
+
+
+@pytest.mark.parametrize("tag", ["vmc_N14", "vmc_N66"])
+def test_vmc_trajectory(oracle_mod, tag):
+    """Variational runs (step1 with idmc = 0): walkers after every dumped step and the six
+    block estimators against the reference."""
+    g = golden(f"dmc_{tag}.npz")
+    s, x, dpsi, sc, irn = _run_reference_trajectory(oracle_mod, g)
+    dumps = set(int(d) for d in g["dumps"])
+    nstep = int(g["nstep"])
+    est = np.zeros(42)
+    for j in range(1, len(g["pop"]) + 1):
+        x, dpsi, sc, irn, est = s.vmc_step1(x, dpsi, sc, irn, est=est)
+        assert len(x) == int(g["pop"][j - 1][1])
+        if j in dumps:
+            ri, rs, rx = _sorted(g[f"d{j}_irn"], g[f"d{j}_sc"], g[f"d{j}_x"])
+            oi, os_, ox = _sorted(irn, sc, x)
+            np.testing.assert_array_equal(oi, ri)
+            np.testing.assert_array_equal(os_[:, 4], rs[:, 4])
+            np.testing.assert_allclose(os_[:, 1], rs[:, 1], rtol=1e-10)
+            np.testing.assert_allclose(ox, rx, rtol=0, atol=1e-10)
+        if j % nstep == 0:
+            e = est.reshape(14, 3)
+            ref = g["est"][j // nstep - 1]
+            for i in range(6):
+                np.testing.assert_allclose(e[i, 0] / e[i, 2], ref[i, 0], rtol=1e-9, atol=1e-30)
+            est = np.zeros(42)
+
+
+@pytest.mark.parametrize("tag,n,norb", [("N14", 14, 3), ("N66", 66, 5)])
+def test_optimizer_update(oracle_mod, tag, n, norb):
+    """calder + updatepar on configurations: mean energy and new parameters."""
+    g = golden(f"opt_{tag}.npz")
+    s = oracle_mod.System(npart=n, norb=norb)
+    ene, par = oracle_mod.calder_updatepar(s, g["x"], float(g["delta"]), float(g["eps"]))
+    assert ene == pytest.approx(float(g["ene"]), rel=1e-10)
+    np.testing.assert_allclose(par, g["par"], rtol=1e-7, atol=1e-12)
