@@ -6,7 +6,7 @@ module dmc_gpu
    integer(c_int), parameter :: DMC_OK=0, DMC_ERR_ARG=1, DMC_ERR_HIP=2, DMC_ERR_OVERFLOW=3, &
       DMC_ERR_SETUP=4, DMC_ERR_NODEVICE=5
    integer(c_int32_t), parameter :: DMC_FP64=0, DMC_MIXED=1
-   integer(c_int32_t), parameter :: DMC_ALLPARTICLE=0, DMC_SINGLEPARTICLE=1
+   integer(c_int32_t), parameter :: DMC_ALLPARTICLE=0, DMC_SINGLEPARTICLE=1, DMC_VMC=2
 
    type, bind(c) :: dmc_params
       integer(c_int32_t) :: npart, norb, ntab, fn, units, nwalk, capacity, precision, &
@@ -117,6 +117,16 @@ module dmc_gpu
          real(c_double), intent(in) :: x(*)
          real(c_double), intent(out) :: dlogpsi(*)
          type(c_ptr), value :: delocal
+      end function
+      integer(c_int) function dmc_opt_accumulate(ctx) bind(c, name='dmc_opt_accumulate')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function dmc_opt_sums(ctx, sums, reset) bind(c, name='dmc_opt_sums')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: sums(*)
+         integer(c_int32_t), value :: reset
       end function
       integer(c_int) function dmc_sync(ctx) bind(c, name='dmc_sync')
          import :: c_int, c_ptr

@@ -5,7 +5,9 @@
 ! writes the final walkers to fort.9 (dmcbcs.f90:548-567).
 !
 ! Usage:  dmcbcs_gpu [fp64|mixed] [allparticle|singleparticle] [krefresh] < deck
-! The defaults (fp64 allparticle) propagate exactly as the reference's step1.
+! The defaults (fp64 allparticle) propagate exactly as the reference's step1.  Decks with
+! idmc = 0 run variational Monte Carlo, idmc = -1 adds the parameter optimisation
+! (calder after every block, updatepar after each of optiter iterations).
 program dmcbcs_gpu
    use engine
    use potential
@@ -13,10 +15,13 @@ program dmcbcs_gpu
    use stack
    use step
    use estimator
+   use optimizer
    implicit none
    integer(kind=i8) :: irn,irn0
    integer(kind=i4) :: npart,nwalk,idmc,neq,nav,nstep,ndim,ntab,nbin(2)
-   integer(kind=i4) :: i,j,k,it,nt,n1,idum,prec,movet,kref
+   integer(kind=i4) :: i,j,k,it,nt,n1,idum,prec,movet,kref,optiter,iters
+   real(kind=r8) :: ene
+   logical :: vmc
    real(kind=r8) :: hbar,dt,etrial,etrial0,popup,popdn,dcut,dobal,wtcut,mass(2)
    real(kind=r8) :: el,mu,eunit,kunit,rho,pi,tau,tau0
    logical :: isite,units,fn,euclid,empty
@@ -69,6 +74,11 @@ program dmcbcs_gpu
    read (5,*) nbin(2),mass(2)
    mu=mass(1)*mass(2)/(mass(1)+mass(2))
    etrial0=etrial
+   vmc=idmc.eq.0.or.idmc.eq.-1
+   if (vmc) then
+      prec=DMC_FP64
+      movet=DMC_VMC
+   endif
    call engine_deck(nwalk,etrial0,units,popup,popdn,prec,movet,kref,int(10*nwalk,i4))
 
 ! module set-up in the order of dmcbcs.f90:142-177
@@ -90,7 +100,20 @@ program dmcbcs_gpu
       read (5,*) idum
    enddo
    euclid=.false.
+   optiter=1
+   if (idmc.eq.-1) call setupopt(npart,optiter)
    call setstep(idmc,ndim,npart,nbin,hbar,dt,etrial,el,mass,wtcut,dcut,eunit,fn)
+   if (vmc) then
+! the variational estimators of dmcbcs.f90:180-186
+      call setestnum(6)
+      allocate(answer(6))
+      call addest(1,'total energy')
+      call addest(2,'kinetic energy')
+      call addest(3,'potential energy')
+      call addest(4,'external potential')
+      call addest(5,'jackson-feenberg energy')
+      call addest(6,'acceptance =')
+   else
    call setestnum(14)
    allocate(answer(14))
    call addest(1,'total energy')
@@ -107,6 +130,7 @@ program dmcbcs_gpu
    call addest(12,'wtg =')
    call addest(13,'sign(wtg) =')
    call addest(14,'sign(wtg)-weighted =')
+   endif
    call create(2,int(10*nwalk,i4),npart,ndim,euclid)
 
 ! initial walkers (dmcbcs.f90:237-301): the reference's generation from one random
@@ -132,6 +156,13 @@ program dmcbcs_gpu
    write (6,'(''Configurations at tau ='',t35,f15.10)') tau0
    tau=tau0
 
+! optimisation iterations (dmcbcs.f90:234-539): after the first, the walkers' stored
+! wavefunction values are recomputed with the new parameters (dmcbcs.f90:314-353)
+   do iters=1,optiter
+   if (iters.gt.1) then
+      write (6,'(''optimization iteration number '',i10)') iters
+      call reload_stack
+   endif
 ! blocks of nstep steps (dmcbcs.f90:426-449)
    it=0
    do i=1,nav+neq
@@ -144,10 +175,10 @@ program dmcbcs_gpu
       call cpu_time(timebl0)
       do j=1,nstep
          call step1(1,2,j.eq.nstep)
-         call branch(2,1,1.0_r8)
+         if (idmc.gt.0) call branch(2,1,1.0_r8)
          nt=numstack(1)
          if (nt.gt.popup*nwalk.or.nt.lt.popdn*nwalk) call branch(1,2,1.0_r8*nwalk/nt)
-         tau=tau+dt
+         if (.not.vmc) tau=tau+dt
       enddo
       call cpu_time(timebl)
       nt=numstack(1)
@@ -157,6 +188,10 @@ program dmcbcs_gpu
       answer=resstring(tau)
       write (6,'(a90)') (answer(k),k=1,size(answer))
       write (6,'(''Time for block ='',f10.3,'' seconds'')') timebl-timebl0
+! dmcbcs.f90:492-503: calder over the walkers after every block past equilibration
+      if (idmc.eq.-1.and.i.gt.neq) call calder
+   enddo
+   if (idmc.eq.-1) call updatepar(ene)
    enddo
 
 ! final walkers to fort.9 (dmcbcs.f90:548-567)

@@ -7,6 +7,7 @@
 !   stack        type(walker), create, push, pop, numstack   (stack.f90:6-119)
 !   step         setstep, step1, branch, writewtisto         (step.f90:11-334)
 !   estimator    setestnum, addest, zerest, update, resstring (estimator.f90:19-149)
+!   optimizer    setupopt, calder, updatepar                  (optimizer.f90:14-169)
 ! All computation runs on the device through the C-ABI (module dmc_gpu); the walker
 ! population stays in device memory.  The deck values each reference module reads from
 ! unit 5 are read here in the same order and collected into one dmc_params record, and the
@@ -159,6 +160,17 @@ contains
    endif
    end subroutine flush_stack
 
+! the resident walkers evaluated again with the current parameters (positions and random
+! states kept; weight 1, wt0 = sign), as the set-up pass of dmcbcs.f90:314-353
+   subroutine reload_stack
+   integer(c_int32_t) :: nw
+   call dmc_check(dmc_read_walkers(engine_ctx(),cap,nw,c_loc(hx),c_null_ptr,c_null_ptr, &
+      c_loc(hirn)),'reload')
+   call dmc_check(dmc_load_walkers(ctx,nw,hx,hirn),'reload')
+   nhost=0
+   downloaded=.false.
+   end subroutine reload_stack
+
    function numstack(i)
    integer(kind=i4) :: numstack,i
    integer(c_int32_t) :: nw
@@ -295,7 +307,8 @@ contains
    real(kind=r8) :: hbar,dt,etrial,el,mass(:),wtcut,dcut,eunit
    logical :: fn
    idmc=idmcin
-   if (idmc.ne.1) error stop 'setstep: the engine propagates DMC (idmc = 1)'
+   if (idmc.ne.1.and.idmc.ne.0.and.idmc.ne.-1) &
+      error stop 'setstep: the engine runs DMC (idmc = 1) and VMC (idmc = 0, -1)'
    prm%hbar=hbar
    prm%dt=dt
    prm%wtcut=wtcut
@@ -386,3 +399,105 @@ contains
    enddo
    end function resstring
 end module estimator
+
+module optimizer
+   use engine
+   implicit none
+   real(kind=r8), private, save :: delta0,eps
+   integer(kind=i4), private, save :: nparm
+   logical, private, save :: iopt(2)
+contains
+! setupopt (optimizer.f90:14-69): optiter, optimise Jastrow, optimise nodes, delta, eps.
+! The parameters are the determinant's alpha_1..alpha_norb (getdetparam); the cosh Jastrow
+! has none.
+   subroutine setupopt(npart,optiter)
+   integer(kind=i4) :: npart,optiter
+   read (5,*) optiter
+   read (5,*) iopt(1)
+   read (5,*) iopt(2)
+   read (5,*) delta0
+   read (5,*) eps
+   nparm=0
+   if (iopt(2)) nparm=prm%norb
+   write (6,'(''iterations for the optimization'',t40,i10)') optiter
+   write (6,'(''optimize Jastrow'',t40,l10)') iopt(1)
+   write (6,'(''optimize nodes'',t40,l10)') iopt(2)
+   write (6,'(''delta ='',t40,f10.5)') delta0
+   write (6,'(''epsilon ='',t40,f10.5)') eps
+   write (6,'(''number of parameters ='',t40,i10)') nparm
+   end subroutine setupopt
+
+! calder (optimizer.f90:90-112) for every resident walker at once: the device computes
+! d log psi with the adjoint kernel and accumulates the sums
+   subroutine calder
+   call dmc_check(dmc_opt_accumulate(engine_ctx()),'calder')
+   end subroutine calder
+
+! updatepar (optimizer.f90:114-169): f = -2<O E> + 2<O><E>, S = <O O> - <O><O> + eps,
+! parameters += delta S^-1 f; the new parameters go to the wavefunction (setdetparam)
+   subroutine updatepar(ene)
+   real(kind=r8) :: ene
+   real(c_double) :: sums(3+2*(prm%norb+2)+(prm%norb+2)**2)
+   real(kind=r8) :: ovec(nparm),ovecw(nparm),pmat(nparm,nparm),fvec(nparm),dpar(nparm)
+   real(kind=r8) :: n,esum,esum2,eerr
+   integer(kind=i4) :: i,j,np
+   np=prm%norb+2
+   call dmc_check(dmc_opt_sums(engine_ctx(),sums,1_c_int32_t),'updatepar')
+   n=sums(1)
+   esum=sums(2)/n
+   esum2=sums(3)/n
+   eerr=sqrt(abs(esum**2-esum2)/n)
+   ene=esum
+   if (nparm.eq.0) return
+   do i=1,nparm
+      ovec(i)=sums(3+i)/n
+      ovecw(i)=sums(3+np+i)/n
+   enddo
+   do i=1,nparm
+      do j=1,nparm
+         pmat(i,j)=sums(3+2*np+(i-1)*np+j)/n-ovec(i)*ovec(j)
+      enddo
+      pmat(i,i)=pmat(i,i)+eps
+   enddo
+   fvec=-2.0_r8*ovecw+2.0_r8*ovec*esum
+   call solve(nparm,pmat,fvec,dpar)
+   prm%vorb(1:nparm)=prm%vorb(1:nparm)+delta0*dpar
+   call dmc_check(dmc_set_params(engine_ctx(),prm%vorb,prm%bbeta,prm%dbeta),'updatepar')
+   write (6,*) 'eave = ',ene,eerr
+   write (6,'(''new parameters: '')')
+   do i=1,nparm
+      write (6,'(f30.20)') prm%vorb(i)
+   enddo
+   end subroutine updatepar
+
+! x = a^-1 b by Gauss-Jordan elimination with partial pivoting (a is overwritten)
+   subroutine solve(n,a,b,x)
+   integer(kind=i4) :: n
+   real(kind=r8) :: a(n,n),b(n),x(n)
+   integer(kind=i4) :: k,p,i
+   real(kind=r8) :: f,t(n),tb
+   x=b
+   do k=1,n
+      p=k-1+maxloc(abs(a(k:n,k)),1)
+      if (a(p,k).eq.0.0_r8) error stop 'updatepar: singular parameter matrix'
+      if (p.ne.k) then
+         t=a(k,:)
+         a(k,:)=a(p,:)
+         a(p,:)=t
+         tb=x(k)
+         x(k)=x(p)
+         x(p)=tb
+      endif
+      f=1.0_r8/a(k,k)
+      a(k,:)=a(k,:)*f
+      x(k)=x(k)*f
+      do i=1,n
+         if (i.ne.k) then
+            f=a(i,k)
+            a(i,:)=a(i,:)-f*a(k,:)
+            x(i)=x(i)-f*x(k)
+         endif
+      enddo
+   enddo
+   end subroutine solve
+end module optimizer

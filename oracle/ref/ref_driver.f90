@@ -319,8 +319,10 @@ contains
    integer(kind=i4) :: istin,istout,n1,nt,i,j,k,jj,ist
    integer(kind=i8) :: ndump,nin,kk
    integer(kind=i8), allocatable :: dumps(:)
-   real(kind=r8) :: rn(1),dummy,tau,valn,errn,val,err
+   real(kind=r8) :: rn(1),dummy,tau,valn,errn,val,err,ene
    real(kind=r8), allocatable :: xtemp(:)
+   real(kind=r8), pointer :: par(:)
+   integer(kind=i4) :: npar
    logical :: empty
    allocate(xtemp(ndim))
    call openin
@@ -399,7 +401,29 @@ contains
          call result(k,valn,errn,val,err)
          write (21) valn,errn
       enddo
+! dmcbcs.f90:492-503: optimisation runs accumulate calder over the walkers after each block
+      if (idmc.eq.-1.or.idmc.eq.-2) then
+         do while(.true.)
+            call pop(istin,w1,empty)
+            if (empty) exit
+            call hpsi(w1)
+            ene=(-hbar*w1%d2psi+w1%v)/eunit/npart
+            call calder(w1,ene)
+            call push(istout,w1)
+         enddo
+         istout=istin
+         istin=3-istout
+      endif
    enddo
+! dmcbcs.f90:538: the parameter update at the end of the optimisation iteration
+   if (idmc.eq.-1.or.idmc.eq.-2) then
+      call updatepar(ene)
+      call updateparams
+      call getdetparam(npar,par)
+      write (21) ene
+      write (21) int(npar,i8)
+      write (21) par
+   endif
    close (21)
    end subroutine do_dmc
 

@@ -183,6 +183,10 @@ def dmc_fixture(tag, dumps, keep_x=True, init=None, **deckargs):
             out[f"d{jj}"] = read_dump(r, n)
         if jj % nstep == 0:
             est.append(r.f(28).reshape(14, 2))
+    opt = None
+    if p["idmc"] in (-1, -2):
+        ene = r.f()
+        opt = (ene, r.f(r.i()))
     assert r.done()
     arrs = dict(npart=n, norb=p["norb"], pop=pop, est=np.array(est), deck=deck,
                 dumps=np.asarray(dumps, np.int64), nwalk=p["nwalk"], nav=p["nav"],
@@ -195,6 +199,9 @@ def dmc_fixture(tag, dumps, keep_x=True, init=None, **deckargs):
             arrs[f"{key}_dpsi"] = dpsi
     if init is not None:
         arrs["init_x"], arrs["init_irn"] = init
+    if opt is not None:
+        arrs["opt_ene"], arrs["opt_par"] = opt
+        arrs["idmc"] = p["idmc"]
     np.savez_compressed(os.path.join(HERE, f"dmc_{tag}.npz"), **arrs)
     return out
 
@@ -205,6 +212,9 @@ def main_vmc():
                 nstep=10, dt=0.002, seed=11)
     dmc_fixture("vmc_N66", [0, 1, 3], idmc=0, npart=66, norb=5, nwalk=24, nav=1, nstep=3,
                 dt=0.001, seed=11)
+    # a whole optimisation iteration (VMC blocks with calder after each, then updatepar)
+    dmc_fixture("optrun_N14", [], idmc=-1, npart=14, norb=3, nwalk=64, nav=3, nstep=5,
+                dt=0.002, seed=11)
     opt_fixture(14, 3, random_configs(14, 64, 6014), "N14")
     opt_fixture(66, 5, random_configs(66, 16, 6066), "N66")
 

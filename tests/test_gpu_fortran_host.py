@@ -64,3 +64,24 @@ def test_driver_restart_and_single_particle_mixed(tmp_path):
     assert len(blocks) == int(g["nav"])
     for nt, est in blocks:
         assert nt > 0 and np.all(np.isfinite(est[:, 0]))
+
+
+def test_driver_optimisation_run_matches_reference(tmp_path):
+    """idmc = -1: VMC blocks (the six variational estimators), calder over the walkers after
+    each block, updatepar at the end: the reference's block values and new parameters."""
+    g = golden("dmc_optrun_N14.npz")
+    out = run(str(g["deck"]), tmp_path)
+    lines = out.splitlines()
+    blocks = []
+    for k, line in enumerate(lines):
+        if line.startswith("walker number"):
+            blocks.append([float(e[30:].split()[1]) for e in lines[k + 1:k + 7]])
+    assert len(blocks) == int(g["nav"])
+    for b, vals in enumerate(blocks):
+        for i in (0, 1, 4, 5):
+            np.testing.assert_allclose(vals[i], g["est"][b][i, 0], rtol=1e-8)
+    k = next(i for i, l in enumerate(lines) if l.startswith("new parameters"))
+    par = np.array([float(v) for v in lines[k + 1:k + 1 + len(g["opt_par"])]])
+    np.testing.assert_allclose(par, g["opt_par"], rtol=1e-7, atol=1e-12)
+    ene = float(next(l for l in lines if "eave" in l).split()[2])
+    assert ene == pytest.approx(float(g["opt_ene"]), rel=1e-10)

@@ -195,3 +195,18 @@ def test_optimizer_update(oracle_mod, tag, n, norb):
     ene, par = oracle_mod.calder_updatepar(s, g["x"], float(g["delta"]), float(g["eps"]))
     assert ene == pytest.approx(float(g["ene"]), rel=1e-10)
     np.testing.assert_allclose(par, g["par"], rtol=1e-7, atol=1e-12)
+
+
+def test_optimisation_run(oracle_mod):
+    """A whole optimisation iteration (idmc = -1): VMC blocks, calder over the walkers after
+    each block, updatepar: the reference's mean energy and new parameters."""
+    g = golden("dmc_optrun_N14.npz")
+    s, x, dpsi, sc, irn = _run_reference_trajectory(oracle_mod, g)
+    confs = []
+    for b in range(int(g["nav"])):
+        for j in range(int(g["nstep"])):
+            x, dpsi, sc, irn, _ = s.vmc_step1(x, dpsi, sc, irn)
+        confs.append(x.copy())
+    ene, par = oracle_mod.calder_updatepar(s, np.concatenate(confs), 0.7, 1e-4)
+    assert ene == pytest.approx(float(g["opt_ene"]), rel=1e-10)
+    np.testing.assert_allclose(par, g["opt_par"], rtol=1e-9)
