@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64 * WPG) void k_batch_moves(BatchArgs a, int chunk
 #define DMC_MOVES(NUP)                                                                      \
     {                                                                                       \
         const int m0 = chunk * b.k, m1 = min(m0 + b.k, b.npart);                            \
-        sm_moves_walker<NUP, R>(b.S, sm, w, tid, m0 ? b.B.x : b.A.x, b.B.x,                 \
+        sm_moves_walker_any<NUP, R>(b.S, sm, w, tid, m0 ? b.B.x : b.A.x, b.B.x,             \
                                 m0 ? b.B.irn : b.A.irn, b.B.irn, b.A.sinv, m0, m1, b.T);    \
     }
     DMC_BATCH_LOOP(DMC_MOVES)

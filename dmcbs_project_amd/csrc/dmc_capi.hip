@@ -115,14 +115,17 @@ int upload_tables(dmc_ctx *c) {
     const System &s = c->sys;
     const int n1 = s.ntab + 1;
     std::vector<double4> jt(n1), bt(n1), btb(n1), btd(n1);
-    std::vector<float4> jt32(n1), bt32(n1);
+    std::vector<double4> mt(n1);
+    std::vector<float4> mt32(n1);
     for (int i = 0; i < n1; ++i) {
         jt[i] = make_double4(s.vt[i], s.ut[i], s.dut[i], s.d2ut[i]);
         bt[i] = make_double4(s.bt[i], s.dbt[i], s.d2bt[i], 0.0);
         btb[i] = make_double4(s.bt_b[i], s.dbt_b[i], s.d2bt_b[i], 0.0);
         btd[i] = make_double4(s.bt_d[i], s.dbt_d[i], s.d2bt_d[i], 0.0);
-        jt32[i] = make_float4(float(s.vt[i]), float(s.ut[i]), float(s.dut[i]), float(s.d2ut[i]));
-        bt32[i] = make_float4(float(s.bt[i]), float(s.dbt[i]), float(s.d2bt[i]), 0.0f);
+        // the moves' pair records: beta and the Jastrow on their common grid (both tables
+        // span [0, L/2] in ntab intervals)
+        mt[i] = make_double4(s.bt[i], s.dbt[i], s.ut[i], s.dut[i]);
+        mt32[i] = make_float4(float(s.bt[i]), float(s.dbt[i]), float(s.ut[i]), float(s.dut[i]));
     }
     HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.btb), btb.data(), sizeof(double4) * n1,
                           hipMemcpyHostToDevice, c->st));
@@ -132,9 +135,9 @@ int upload_tables(dmc_ctx *c) {
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.bt), bt.data(), sizeof(double4) * n1,
                           hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.jt32), jt32.data(), sizeof(float4) * n1,
+    HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.mt64), mt.data(), sizeof(double4) * n1,
                           hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.bt32), bt32.data(), sizeof(float4) * n1,
+    HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.mt32), mt32.data(), sizeof(float4) * n1,
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return 0;
@@ -278,13 +281,13 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     }
     fill_devsys(c);
     const int n1 = s.ntab + 1, nq = s.nk - 1;
-    double4 *jt, *bt, *btb, *btd;
-    float4 *jt32, *bt32;
+    double4 *jt, *bt, *btb, *btd, *mt64;
+    float4 *mt32;
     int4 *qm;
     double *wq;
     uint64_t *ja1, *jc1, *ja2, *jc2;
     if (c->alloc(&jt, n1) || c->alloc(&bt, n1) || c->alloc(&btb, n1) || c->alloc(&btd, n1) ||
-        c->alloc(&jt32, n1) || c->alloc(&bt32, n1) ||
+        c->alloc(&mt64, n1) || c->alloc(&mt32, n1) ||
         c->alloc(&qm, nq) || c->alloc(&wq, nq) || c->alloc(&ja1, kMaxDraw) ||
         c->alloc(&jc1, kMaxDraw) || c->alloc(&ja2, kMaxDraw) || c->alloc(&jc2, kMaxDraw) ||
         c->alloc_state(c->A) || c->alloc_state(c->B) || c->alloc(&c->T.count, c->capacity) ||
@@ -301,8 +304,8 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     c->ds.bt = bt;
     c->ds.btb = btb;
     c->ds.btd = btd;
-    c->ds.jt32 = jt32;
-    c->ds.bt32 = bt32;
+    c->ds.mt64 = mt64;
+    c->ds.mt32 = mt32;
     c->ds.qm = qm;
     c->ds.wq = wq;
     c->ds.ja1 = ja1;

@@ -16,8 +16,8 @@ struct DevSys {
         nfac, twopi_l, popup, popdn;
     const double4 *jt;    // Jastrow/potential records {v, u, du, d2u}, [ntab+1]
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
-    const float4 *jt32;   // FP32 copies for the mixed-precision moves
-    const float4 *bt32;
+    const double4 *mt64;  // pair records of the single-particle moves {b, db, u, du}, [ntab+1]
+    const float4 *mt32;   // and their FP32 copy (mixed-precision moves)
     const double4 *btb;   // d/db and d/dd of the beta records (parameter adjoint)
     const double4 *btd;
     const int4 *qm;       // integer q vectors (mx, my, mz, shell), [nq]

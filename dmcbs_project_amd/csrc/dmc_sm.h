@@ -48,17 +48,28 @@ __device__ __forceinline__ R pair_dist(const DevSys &S, const double *xa, const 
     return sqrt_r<R>(r2);
 }
 
-template <typename R, typename V>
-__device__ __forceinline__ R field(const V &v, int k) {
-    return k == 0 ? R(v.x) : k == 1 ? R(v.y) : k == 2 ? R(v.z) : R(v.w);
-}
+template <typename R>
+struct Vec4;
+template <>
+struct Vec4<double> {
+    using T = double4;
+};
+template <>
+struct Vec4<float> {
+    using T = float4;
+};
 
-// R-typed interpolation of two fields of a table record at s (4-point Lagrange,
-// coshpot.f90:172-192); fields (0,1) = (b, db) of the beta table, (1,2) = (u, du) of the
-// Jastrow table
-template <typename R, int F0, int F1>
-__device__ __forceinline__ void tab2(const double4 *t64, const float4 *t32, R s, int ntab, R &x,
-                                     R &y) {
+// The pair table of the moves: record i = {beta, beta', u, u'} at r = i / scale (the beta
+// and Jastrow tables share the grid [0, L/2] of ntab intervals), interpolated with the
+// 4-point Lagrange weights of coshpot.f90:172-192.  Beyond L/2 every field is zero.
+template <typename R>
+__device__ __forceinline__ typename Vec4<R>::T pair_tab(const DevSys &S, R r) {
+    using R4 = typename Vec4<R>::T;
+    R4 o;
+    o.x = o.y = o.z = o.w = R(0);
+    if (!(r <= R(S.rangeb))) return o;
+    const R s = R(S.scaleb) * r;
+    const int ntab = S.ntab;
     int idx = int(s);
     idx = idx > ntab - 2 ? ntab - 2 : idx;
     idx = idx < 1 ? 1 : idx;
@@ -67,35 +78,74 @@ __device__ __forceinline__ void tab2(const double4 *t64, const float4 *t32, R s,
     const R w1 = (t + R(1)) * (t - R(1)) * (t - R(2)) * R(0.5);
     const R w2 = -(t + R(1)) * t * (t - R(2)) * R(0.5);
     const R w3 = (t + R(1)) * t * (t - R(1)) * R(1.0 / 6.0);
-    if constexpr (sizeof(R) == 8) {
-        const double4 a = t64[idx - 1], b = t64[idx], c = t64[idx + 1], d = t64[idx + 2];
-        x = w0 * field<R>(a, F0) + w1 * field<R>(b, F0) + w2 * field<R>(c, F0) + w3 * field<R>(d, F0);
-        y = w0 * field<R>(a, F1) + w1 * field<R>(b, F1) + w2 * field<R>(c, F1) + w3 * field<R>(d, F1);
-    } else {
-        const float4 a = t32[idx - 1], b = t32[idx], c = t32[idx + 1], d = t32[idx + 2];
-        x = w0 * field<R>(a, F0) + w1 * field<R>(b, F0) + w2 * field<R>(c, F0) + w3 * field<R>(d, F0);
-        y = w0 * field<R>(a, F1) + w1 * field<R>(b, F1) + w2 * field<R>(c, F1) + w3 * field<R>(d, F1);
-    }
+    const R4 *tab;
+    if constexpr (sizeof(R) == 8) tab = S.mt64;
+    else tab = S.mt32;
+    const R4 a = tab[idx - 1], b = tab[idx], c = tab[idx + 1], d = tab[idx + 2];
+    o.x = w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x;
+    o.y = w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y;
+    o.z = w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z;
+    o.w = w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w;
+    return o;
 }
 
-// LDS of the move kernel (offsets in doubles)
+// Pair terms of the mover at xa with the partner at xp: the minimum-image difference dx,
+// beta'(r)/r, u'(r)/r and u(r) (r floored at 1e-14 for the Jastrow part, coshpot.f90:167)
+// and, if wanted, beta(r)
+template <typename R>
+__device__ __forceinline__ void pair_terms(const DevSys &S, const double *xa, const double *xp,
+                                           R (&dx)[3], R &tb, R &tj, R &u, R &bv) {
+    const R r = pair_dist<R>(S, xa, xp, dx);
+    const typename Vec4<R>::T v = pair_tab<R>(S, r);
+    const R rj = r > R(1.0e-14) ? r : R(1.0e-14);
+    bv = v.x;
+    tb = v.y * rcp_r<R>(r);
+    tj = v.w * rcp_r<R>(rj);
+    u = v.z;
+}
+
+// v(lane) + v(lane ^ 32) in every lane (the two half-waves exchanged by a permlane swap)
+__device__ __forceinline__ float half_sum(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ double half_sum(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(b & 0xffffffffll), hi = unsigned(b >> 32);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return dbl_of(int(l[0]), int(h[0])) + dbl_of(int(l[1]), int(h[1]));
+}
+
+// LDS of the move kernel (offsets in doubles, every area 16-byte aligned)
 template <int NUP, typename R>
 struct MvLayout {
-    // row stride of the inverse: >= NUP and = 1 (mod 16), so that lanes walking a column of
-    // the inverse hit distinct banks
-    static constexpr int SP = ((NUP + 14) / 16) * 16 + 1, NP = 2 * NUP;
+    static constexpr int NP = 2 * NUP;
     static constexpr int RD = sizeof(R) == 8 ? 1 : 2;  // R values per double
-    int xs, qc, qm, sv, ph, phn, pw, buf, total;
+    // row stride (complex values) of the phase planes P[q][p] and of the transposition
+    // area: odd, so that lanes reading one partner at different q, or one row of the
+    // transposition each, hit distinct banks
+    static constexpr int ST = NUP | 1;
+    // the column broadcast is split in two halves (partners [0, HB) and [HB, NUP)), each
+    // starting on a 16-byte boundary
+    static constexpr int HB = (NUP + 1) / 2, HBP = (HB + 3) / 4 * 4;
+    static constexpr int NB = (NUP + 3) / 4 * 4;
+    static __host__ __device__ constexpr int dbl(int nr) { return ((nr + RD - 1) / RD + 1) / 2 * 2; }
+    int xs, pu, pd, qk, qm, wb, fb, ub, total;
     __host__ __device__ explicit MvLayout(int nq) {
-        xs = 0;
-        qc = xs + 3 * NP;
-        qm = qc + (5 * nq + RD - 1) / RD;
-        sv = qm + 2 * nq;
-        ph = sv + (NUP * SP + RD - 1) / RD;
-        phn = ph + (2 * NP * nq + RD - 1) / RD;
-        pw = phn + (2 * nq + RD - 1) / RD;      // exp(i m theta_d), d < 3, m = 0..4
-        buf = pw + (2 * 15 + RD - 1) / RD;
-        total = buf + (3 * SP + RD - 1) / RD;
+        xs = 0;                                  // positions, [NP][3] FP64
+        const int plane = dbl(2 * nq * ST);      // phases of one spin, [nq][ST] complex
+        const int trans = dbl(NUP * ST);         // transposition of the inverse (overlay)
+        pu = xs + 3 * NP;
+        pd = pu + plane;
+        const int pend = pu + (2 * plane > trans ? 2 * plane : trans);
+        qk = pend;                               // per q: w_q, -w_q q (R)
+        qm = qk + dbl(4 * nq);                   // integer q vectors (int4)
+        wb = qm + 2 * nq;                        // inverse column, two halves (R)
+        fb = wb + dbl(2 * HBP);                  // new pair values (R)
+        ub = fb + dbl(NB);                       // w_q exp(i q.x') (R)
+        total = ub + dbl(2 * nq + 4);
     }
 };
 
@@ -122,68 +172,16 @@ __device__ __forceinline__ void powers(const DevSys &S, double x, R (&c)[5], R (
     }
 }
 
-// product of the three factors exp(i m_d theta_d), read from a power table pw[d][m]
-template <typename R>
-__device__ __forceinline__ typename Vec2<R>::T qphase(const typename Vec2<R>::T *pw,
-                                                      const int4 m) {
-    const int mm[3] = {m.x, m.y, m.z};
-    R c[3], s[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int am = mm[d] < 0 ? -mm[d] : mm[d];
-        const typename Vec2<R>::T v = pw[5 * d + am];
-        c[d] = v.x;
-        s[d] = mm[d] < 0 ? -v.y : v.y;
-    }
-    const R cxy = c[0] * c[1] - s[0] * s[1], sxy = s[0] * c[1] + c[0] * s[1];
-    typename Vec2<R>::T z;
-    z.x = cxy * c[2] - sxy * s[2];
-    z.y = sxy * c[2] + cxy * s[2];
-    return z;
-}
-
-// moves m0..m1-1 of walker w: positions and RNG state from xin/irn_in, the stored inverse
-// sinv_g; results to xout/irn_out (LDS at sm, MvLayout)
+// phases exp(i q.r) of every particle into the planes pu (up) / pd (down), [q][ST]
 template <int NUP, typename R>
-__device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int w, int tid,
-                                                const double *__restrict__ xin, double *xout,
-                                                const uint64_t *irn_in, uint64_t *irn_out,
-                                                char *sinv_g, int m0, int m1, const DevStep &T) {
+__device__ __forceinline__ void move_phases(const DevSys &S, const double *xs, const int4 *qm,
+                                            typename Vec2<R>::T *pu, typename Vec2<R>::T *pd,
+                                            int tid) {
     using R2 = typename Vec2<R>::T;
-    const MvLayout<NUP, R> X(S.nq);
-    constexpr int M = 6 * NUP, NP = 2 * NUP, SP = MvLayout<NUP, R>::SP;
+    constexpr int NP = 2 * NUP, ST = NUP | 1;
     const int nq = S.nq;
-    double *xs = sm + X.xs;
-    R *qc = reinterpret_cast<R *>(sm + X.qc);
-    int4 *qm = reinterpret_cast<int4 *>(sm + X.qm);
-    R *sv = reinterpret_cast<R *>(sm + X.sv);
-    R2 *ph = reinterpret_cast<R2 *>(sm + X.ph);
-    R2 *phn = reinterpret_cast<R2 *>(sm + X.phn);
-    R2 *pw = reinterpret_cast<R2 *>(sm + X.pw);
-    R *phinew = reinterpret_cast<R *>(sm + X.buf);
-    R *colbuf = phinew + SP;
-    R *fbuf = colbuf + SP;
-
-    for (int q = tid; q < nq; q += 64) {
-        const int4 m = S.qm[q];
-        const double wq = S.wq[q], tp = S.twopi_l;
-        qm[q] = m;
-        qc[5 * q] = R(wq);
-        qc[5 * q + 1] = R(-wq * tp * m.x);
-        qc[5 * q + 2] = R(-wq * tp * m.y);
-        qc[5 * q + 3] = R(-wq * tp * m.z);
-    }
-    for (int t = tid; t < M; t += 64) xs[t] = wrapd(xin[size_t(w) * M + t], S);
-    {
-        const R *g = reinterpret_cast<const R *>(sinv_g + size_t(w) * sinv_bytes(S));
-        for (int e = tid; e < NUP * NUP; e += 64) {
-            const int j = e / NUP, i = e - j * NUP;
-            sv[j * SP + i] = g[e];
-        }
-    }
-    wave_sync();
-    // q-major phases ph[q * NP + p] of every particle (lanes over particles)
     for (int p = tid; p < NP; p += 64) {
+        R2 *pl = p < NUP ? pu + p : pd + (p - NUP);
         R c[3][5], sn[3][5];
 #pragma unroll
         for (int d = 0; d < 3; ++d) powers<R>(S, xs[3 * p + d], c[d], sn[d]);
@@ -210,8 +208,88 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             R2 z;
             z.x = cxy * cq[2] - sxy * sq[2];
             z.y = sxy * cq[2] + cxy * sq[2];
-            ph[q * NP + p] = z;
+            pl[q * ST] = z;
         }
+    }
+}
+
+// Moves m0..m1-1 of walker w: positions and RNG state from xin/irn_in, the stored inverse
+// sinv_g (dense [j][i] = Sinv(j, i)); results to xout/irn_out (LDS at sm, MvLayout).
+//
+// Mover a with the n opposite-spin partners b.  Lane k < n holds column k of the inverse
+// in registers while up particles move (Sinv(b, k), b = 0..n-1) and row k while down
+// particles move (Sinv(k, b)): in both halves the mover's determinant weights are
+// w_b = "column a" of what the lanes hold, and the Sherman-Morrison step is the same
+// rank-1 update.  The plane-wave parts of the drift at the old and the proposed position
+// and of the ratio go through the projections X_q = sum_b w_b exp(i q.r_b) (64 lanes: one
+// real component of one q per lane, the partners split over the two half-waves):
+//   sum_b w_b grad_a cos(q.(r_a - r_b)) = -q Im(exp(i q.r_a) conj(X_q)).
+// Lanes over the partners add the radial (beta) and Jastrow terms from one table lookup
+// per pair, and the new pair values phi(r'_a - r_b) that the update needs.  The pair terms
+// of the next mover at its current position are formed while the current move is
+// evaluated: moves of one spin never change them.
+template <int NUP, typename R, int NQC>
+__device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int w, int tid,
+                                                const double *__restrict__ xin, double *xout,
+                                                const uint64_t *irn_in, uint64_t *irn_out,
+                                                char *sinv_g, int m0, int m1, const DevStep &T) {
+    using R2 = typename Vec2<R>::T;
+    using L = MvLayout<NUP, R>;
+    const L X(S.nq);
+    constexpr int M = 6 * NUP, ST = L::ST, HB = L::HB, HBP = L::HBP;
+    const int nq = S.nq, no = 2 * nq;
+    double *xs = sm + X.xs;
+    R2 *pu = reinterpret_cast<R2 *>(sm + X.pu);
+    R2 *pd = reinterpret_cast<R2 *>(sm + X.pd);
+    R *tr = reinterpret_cast<R *>(sm + X.pu);
+    R *qk = reinterpret_cast<R *>(sm + X.qk);
+    int4 *qm = reinterpret_cast<int4 *>(sm + X.qm);
+    R *wb = reinterpret_cast<R *>(sm + X.wb);
+    R *fb = reinterpret_cast<R *>(sm + X.fb);
+    R *ub = reinterpret_cast<R *>(sm + X.ub);
+
+    for (int q = tid; q < nq; q += 64) {
+        const int4 m = S.qm[q];
+        const double wq = S.wq[q], tp = S.twopi_l;
+        qm[q] = m;
+        qk[4 * q] = R(wq);
+        qk[4 * q + 1] = R(-wq * tp * m.x);
+        qk[4 * q + 2] = R(-wq * tp * m.y);
+        qk[4 * q + 3] = R(-wq * tp * m.z);
+    }
+    for (int t = tid; t < M; t += 64) xs[t] = wrapd(xin[size_t(w) * M + t], S);
+    wave_sync();
+    move_phases<NUP, R>(S, xs, qm, pu, pd, tid);
+
+    // projection lanes: output o = (lane & 31) + 32 c is component (o & 1) of q = o >> 1
+    const int h = tid >> 5, ol = tid & 31;
+    R cw[NQC], cg[NQC][3];
+    int4 cm[NQC];
+#pragma unroll
+    for (int c = 0; c < NQC; ++c) {
+        const int o = ol + 32 * c, q = o >> 1;
+        const bool ok = o < no;
+        cw[c] = ok ? qk[4 * q] : R(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cg[c][d] = ok ? qk[4 * q + 1 + d] : R(0);
+        cm[c] = ok ? qm[q] : make_int4(0, 0, 0, 0);
+    }
+
+    // the inverse: column (up movers) or row (down movers) k = lane
+    bool colmode = m0 < NUP;
+    R Ac[NUP];
+    {
+        const R *g = reinterpret_cast<const R *>(sinv_g + size_t(w) * sinv_bytes(S));
+        const int k = tid < NUP ? tid : 0;
+#pragma unroll
+        for (int b = 0; b < NUP; ++b) Ac[b] = g[colmode ? b * NUP + k : k * NUP + b];
+    }
+    // partner position of lane b (the partners do not move while one spin moves)
+    double xp[3];
+    {
+        const int pb = (colmode ? NUP : 0) + (tid < NUP ? tid : 0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xp[d] = xs[3 * pb + d];
     }
     wave_sync();
 
@@ -219,41 +297,86 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
     const R alpha0 = R(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
-    const int r0 = tid / NUP, c0 = tid - (tid / NUP) * NUP;
+    // pair terms of the mover at its current position (lanes < NUP)
+    R ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};
+    bool have_old = false;
     for (int ip = m0; ip < m1; ++ip) {
+        if (ip == NUP && colmode) {
+            // down movers from here: rows of the inverse into the lanes through LDS (over
+            // the phase planes, rebuilt afterwards), the up particles become the partners
+            if (tid < NUP) {
+#pragma unroll
+                for (int b = 0; b < NUP; ++b) tr[b * ST + tid] = Ac[b];
+            }
+            wave_sync();
+            if (tid < NUP) {
+#pragma unroll
+                for (int b = 0; b < NUP; ++b) Ac[b] = tr[tid * ST + b];
+            }
+            wave_sync();
+            move_phases<NUP, R>(S, xs, qm, pu, pd, tid);
+            const int pb = tid < NUP ? tid : 0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xp[d] = xs[3 * pb + d];
+            colmode = false;
+            have_old = false;
+            wave_sync();
+        }
         const bool up = ip < NUP;
         const int a = up ? ip : ip - NUP;
+        R2 *Pm = up ? pu : pd;        // phases of the mover's spin
+        const R2 *Pp = up ? pd : pu;  // phases of the partners
         const double xa[3] = {xs[3 * ip], xs[3 * ip + 1], xs[3 * ip + 2]};
-        // old gradient: lanes b < NUP over the opposite-spin partners
-        R go0 = 0, go1 = 0, go2 = 0, uo = 0, wgt = 0;
-        int pb = 0;
-        if (tid < NUP) {
-            const int b = tid;
-            pb = up ? NUP + b : b;
-            wgt = up ? sv[b * SP + a] : sv[a * SP + b];
-            R dx[3];
-            const R ro = pair_dist<R>(S, xa, xs + 3 * pb, dx);
-            // table values first, so that their loads overlap the plane-wave sum
-            R bv = 0, db = 0, uu = 0, du = 0;
-            const R rj = ro > R(1.0e-14) ? ro : R(1.0e-14);
-            if (!S.nobeta && ro <= R(S.rangeb)) tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * ro, S.ntab, bv, db);
-            if (rj < R(S.range)) tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
-#pragma unroll 4
-            for (int q = 0; q < nq; ++q) {
-                const R2 u = ph[q * NP + ip], v = ph[q * NP + pb];
-                const R sab = u.y * v.x - u.x * v.y;
-                go0 += qc[5 * q + 1] * sab;
-                go1 += qc[5 * q + 2] * sab;
-                go2 += qc[5 * q + 3] * sab;
-            }
-            const R t = db * rcp_r<R>(ro), tj = du * rcp_r<R>(rj);
-            uo = uu;
-            // determinant part weighted by the inverse, plus the Jastrow part
-            go0 = wgt * (go0 + t * dx[0]) + dx[0] * tj;
-            go1 = wgt * (go1 + t * dx[1]) + dx[1] * tj;
-            go2 = wgt * (go2 + t * dx[2]) + dx[2] * tj;
+        // the mover's determinant weights w_b to every lane
+        if (tid == a) {
+#pragma unroll
+            for (int b = 0; b < NUP; ++b) wb[b < HB ? b : HBP + b - HB] = Ac[b];
         }
-        const double go[3] = {double(wave_sum(go0)), double(wave_sum(go1)), double(wave_sum(go2))};
+        wave_sync();
+        R wgt = R(0);
+        if (tid < NUP) wgt = wb[tid < HB ? tid : HBP + tid - HB];
+        if (!have_old && tid < NUP) {
+            R bv;
+            pair_terms<R>(S, xa, xp, odx, ob, oj, ou, bv);
+        }
+        // projections X_o, partners [h HB, h HB + HB) in half-wave h
+        R Xo[NQC];
+#pragma unroll
+        for (int c = 0; c < NQC; ++c) {
+            const int o = ol + 32 * c;
+            R acc = R(0);
+            if (o < no) {
+                const R *pl = reinterpret_cast<const R *>(Pp + (o >> 1) * ST) + (o & 1);
+                const R *wh = wb + h * HBP;
+                const int b0 = h * HB;
+#pragma unroll
+                for (int bb = 0; bb < HB; ++bb)
+                    if (b0 + bb < NUP) acc += wh[bb] * pl[2 * (b0 + bb)];
+            }
+            Xo[c] = half_sum(acc);
+        }
+        // drift at the current position
+        R g0 = R(0), g1 = R(0), g2 = R(0);
+        if (tid < NUP) {
+            const R f = wgt * ob + oj;
+            g0 = f * odx[0];
+            g1 = f * odx[1];
+            g2 = f * odx[2];
+        }
+        if (h == 0) {
+#pragma unroll
+            for (int c = 0; c < NQC; ++c) {
+                const int o = ol + 32 * c;
+                if (o < no) {
+                    const R2 u = Pm[(o >> 1) * ST + a];
+                    const R t = (o & 1) ? -u.x * Xo[c] : u.y * Xo[c];
+                    g0 += cg[c][0] * t;
+                    g1 += cg[c][1] * t;
+                    g2 += cg[c][2] * t;
+                }
+            }
+        }
+        const double go[3] = {double(wave_sum(g0)), double(wave_sum(g1)), double(wave_sum(g2))};
         // gaussian(3) then the acceptance draw: five ran1 draws per move
         double u[5];
 #pragma unroll
@@ -284,55 +407,70 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             dxm[d] = dro[d] + sig * g3[d];
             xn[d] = wrapr(xa[d] + dxm[d], S);
         }
-        // phases of the proposed position: powers per axis (lanes 0..2), then lanes over q
-        if (tid < 3) {
-            R c[5], sn[5];
-            powers<R>(S, xn[tid], c, sn);
+        // phases of the proposed position, one q per lane pair: the argument reduced to
+        // a fraction of a turn in FP64
+        R unc[NQC], uns[NQC];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                pw[5 * tid + k].x = c[k];
-                pw[5 * tid + k].y = sn[k];
+        for (int c = 0; c < NQC; ++c) {
+            const int o = ol + 32 * c;
+            const int4 m = cm[c];
+            double turns = (m.x * xn[0] + m.y * xn[1] + m.z * xn[2]) * S.eli;
+            turns -= rint(turns);
+            R sn, cs;
+            if constexpr (sizeof(R) == 8) {
+                sincos(2.0 * kPi * turns, &sn, &cs);
+            } else {
+                sincosf(float(2.0 * kPi * turns), &sn, &cs);
             }
+            unc[c] = cs;
+            uns[c] = sn;
+            if (h == 0 && o < no) ub[o] = cw[c] * ((o & 1) ? sn : cs);
         }
         wave_sync();
-        if (tid < nq) phn[tid] = qphase<R>(pw, qm[tid]);
-        wave_sync();
-        // pair values and gradients at the proposed position
+        // pair values at the proposed position (lanes < NUP)
         double rho_part = 0.0;
-        R gn0 = 0, gn1 = 0, gn2 = 0, jn0 = 0, jn1 = 0, jn2 = 0, dU = 0;
+        R gn0 = 0, gn1 = 0, gn2 = 0, jn0 = 0, jn1 = 0, jn2 = 0, dU = 0, fnew = 0;
         if (tid < NUP) {
-            R dx[3];
-            const R rn = pair_dist<R>(S, xn, xs + 3 * pb, dx);
-            R bv = 0, db = 0, uu = 0, du = 0;
-            const R rj = rn > R(1.0e-14) ? rn : R(1.0e-14);
-            if (!S.nobeta && rn <= R(S.rangeb)) tab2<R, 0, 1>(S.bt, S.bt32, R(S.scaleb) * rn, S.ntab, bv, db);
-            if (rj < R(S.range)) tab2<R, 1, 2>(S.jt, S.jt32, R(S.scale) * rj, S.ntab, uu, du);
-            R f = 0, g0 = 0, g1 = 0, g2 = 0;
+            R dx[3], tb, tj, uu, bv;
+            pair_terms<R>(S, xn, xp, dx, tb, tj, uu, bv);
+            R f = alpha0 + bv;
+            const R2 *pp = Pp + tid;
 #pragma unroll 4
             for (int q = 0; q < nq; ++q) {
-                const R2 un = phn[q], v = ph[q * NP + pb];
-                const R cab = un.x * v.x + un.y * v.y;
-                const R sab = un.y * v.x - un.x * v.y;
-                f += qc[5 * q] * cab;
-                g0 += qc[5 * q + 1] * sab;
-                g1 += qc[5 * q + 2] * sab;
-                g2 += qc[5 * q + 3] * sab;
+                const R2 v = pp[q * ST];
+                f += ub[2 * q] * v.x + ub[2 * q + 1] * v.y;
             }
-            f += alpha0 + bv;
-            const R t = db * rcp_r<R>(rn);
-            g0 += t * dx[0];
-            g1 += t * dx[1];
-            g2 += t * dx[2];
-            phinew[tid] = f;
+            fnew = f;
             rho_part = double(wgt) * double(f);
-            gn0 = wgt * g0;
-            gn1 = wgt * g1;
-            gn2 = wgt * g2;
-            const R tj = du * rcp_r<R>(rj);
-            jn0 = dx[0] * tj;
-            jn1 = dx[1] * tj;
-            jn2 = dx[2] * tj;
-            dU = uu - uo;
+            const R wt = wgt * tb;
+            gn0 = wt * dx[0];
+            gn1 = wt * dx[1];
+            gn2 = wt * dx[2];
+            jn0 = tj * dx[0];
+            jn1 = tj * dx[1];
+            jn2 = tj * dx[2];
+            dU = uu - ou;
+        }
+        if (h == 0) {
+#pragma unroll
+            for (int c = 0; c < NQC; ++c) {
+                const int o = ol + 32 * c;
+                if (o < no) {
+                    const R t = (o & 1) ? -unc[c] * Xo[c] : uns[c] * Xo[c];
+                    gn0 += cg[c][0] * t;
+                    gn1 += cg[c][1] * t;
+                    gn2 += cg[c][2] * t;
+                }
+            }
+        }
+        // pair terms of the next mover of this spin at its current position
+        have_old = false;
+        if (ip + 1 < m1 && ip + 1 != NUP) {
+            if (tid < NUP) {
+                R bv;
+                pair_terms<R>(S, xs + 3 * (ip + 1), xp, odx, ob, oj, ou, bv);
+            }
+            have_old = true;
         }
         const double rho = wave_sum(rho_part);  // FP64 sum: the node test uses its sign
         const double gn[3] = {double(wave_sum(gn0)), double(wave_sum(gn1)), double(wave_sum(gn2))};
@@ -340,54 +478,49 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         const double dlogj = double(wave_sum(dU));
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
-            double logp = 2.0 * (log(fabs(rho)) + dlogj);
             const double irho = 1.0 / rho;
+            double lq = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 double drn = (gn[d] * irho + jn[d]) * sig2;
                 if (S.cutdrift) drn = fmin(S.dcut, fmax(-S.dcut, drn));
-                logp += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]) * half_isig2;
+                lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
             }
-            accept = u[4] < exp(fmin(0.0, logp));
+            if constexpr (sizeof(R) == 8) {
+                const double logp = 2.0 * (log(fabs(rho)) + dlogj) + lq * half_isig2;
+                accept = u[4] < exp(fmin(0.0, logp));
+            } else {
+                const float logp = 2.0f * (logf(fabsf(float(rho))) + float(dlogj)) +
+                                   float(lq * half_isig2);
+                accept = u[4] < double(expf(fminf(0.0f, logp)));
+            }
         }
         if (accept) {
             ++nacc;
             const R irr = R(1.0 / rho);
-            // capture column a (up mover) or row a (down mover) of the inverse
-            if (tid < NUP) colbuf[tid] = up ? sv[tid * SP + a] : sv[a * SP + tid];
+            if (tid < NUP) fb[tid] = fnew;
             wave_sync();
             if (tid < NUP) {
-                const int k = tid;
-                R z = 0;
-                if (up) {
+                R z = R(0);
 #pragma unroll
-                    for (int b = 0; b < NUP; ++b) z += phinew[b] * sv[b * SP + k];
-                } else {
+                for (int b = 0; b < NUP; ++b) z += fb[b] * Ac[b];
+                const R fz = (z - (tid == a ? R(1) : R(0))) * irr;
 #pragma unroll
-                    for (int b = 0; b < NUP; ++b) z += sv[k * SP + b] * phinew[b];
-                }
-                fbuf[k] = (z - (k == a ? R(1) : R(0))) * irr;
+                for (int b = 0; b < NUP; ++b) Ac[b] -= wb[b < HB ? b : HBP + b - HB] * fz;
             }
-            wave_sync();
-            // rank-1 update over the n x n elements, 64 at a time; (r, c) of element
-            // e0 + tid advanced by 64 = kq n + kr without a division
-            {
-                constexpr int kq = 64 / NUP, kr = 64 % NUP;
-                int r = r0, c = c0;
+            if (tid < 3) xs[3 * ip + tid] = xn[tid];
+            if (h == 0) {
 #pragma unroll
-                for (int e0 = 0; e0 < NUP * NUP; e0 += 64) {
-                    if (e0 + tid < NUP * NUP)
-                        sv[r * SP + c] -= up ? colbuf[r] * fbuf[c] : fbuf[r] * colbuf[c];
-                    r += kq;
-                    c += kr;
-                    if (c >= NUP) {
-                        c -= NUP;
-                        ++r;
+                for (int c = 0; c < NQC; ++c) {
+                    const int o = ol + 32 * c;
+                    if (o < no && !(o & 1)) {
+                        R2 z;
+                        z.x = unc[c];
+                        z.y = uns[c];
+                        Pm[(o >> 1) * ST + a] = z;
                     }
                 }
             }
-            if (tid < 3) xs[3 * ip + tid] = xn[tid];
-            if (tid < nq) ph[tid * NP + ip] = phn[tid];
         }
         wave_sync();
     }
@@ -400,6 +533,22 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                   (unsigned long long)(m1 - m0));
     }
 }
+
+// the move kernel's chunk count of projection outputs (32 per chunk) for nq plane waves
+__host__ __device__ inline int mv_chunks(int nq) { return nq <= 16 ? 1 : 4; }
+
+template <int NUP, typename R>
+__device__ __forceinline__ void sm_moves_walker_any(const DevSys &S, double *sm, int w, int tid,
+                                                    const double *__restrict__ xin, double *xout,
+                                                    const uint64_t *irn_in, uint64_t *irn_out,
+                                                    char *sinv_g, int m0, int m1,
+                                                    const DevStep &T) {
+    if (S.nq <= 16)
+        sm_moves_walker<NUP, R, 1>(S, sm, w, tid, xin, xout, irn_in, irn_out, sinv_g, m0, m1, T);
+    else
+        sm_moves_walker<NUP, R, 4>(S, sm, w, tid, xin, xout, irn_in, irn_out, sinv_g, m0, m1, T);
+}
+
 
 // FP64 rebuild of the inverse of positions x (every krefresh moves)
 template <int NUP>

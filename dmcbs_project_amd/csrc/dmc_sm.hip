@@ -28,7 +28,7 @@
 namespace dmc {
 
 // Kernels of one system: walker w = workgroup index, one 64-lane wave per walker.
-template <int NUP, typename R>
+template <int NUP, typename R, int NQC>
 __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restrict__ xin,
                                                  double *xout, const uint64_t *irn_in,
                                                  uint64_t *irn_out, char *sinv_g, int m0,
@@ -36,8 +36,8 @@ __global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restr
     extern __shared__ double sm[];
     const int w = blockIdx.x;
     if (w >= T.nt[0]) return;
-    sm_moves_walker<NUP, R>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g, m0, m1,
-                            T);
+    sm_moves_walker<NUP, R, NQC>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g, m0,
+                                 m1, T);
 }
 
 // FP64 rebuild of the inverse of positions x (every krefresh moves)
@@ -62,20 +62,27 @@ __global__ __launch_bounds__(64) void k_sm_end(DevSys S, DevState A, DevState B,
     sm_end_walker<NUP>(S, sm, w, threadIdx.x, A, B, T, opcomp);
 }
 
+template <int NUP, typename R, int NQC>
+int sm_moves_tq(const DevSys &S, const double *xin, double *xout, const uint64_t *irn_in,
+                uint64_t *irn_out, char *sinv, int m0, int m1, const DevStep &T, int grid,
+                hipStream_t st) {
+    const size_t lds = mv_lds_bytes<NUP, R>(S.nq);
+    hipLaunchKernelGGL((k_sm_moves<NUP, R, NQC>), dim3(grid), dim3(64), lds, st, S, xin, xout,
+                       irn_in, irn_out, sinv, m0, m1, T);
+    return hipGetLastError();
+}
+
 template <int NUP>
 int sm_moves_t(const DevSys &S, const double *xin, double *xout, const uint64_t *irn_in,
                uint64_t *irn_out, char *sinv, int m0, int m1, const DevStep &T, int grid,
                hipStream_t st) {
-    if (S.sinv_f32) {
-        const size_t lds = mv_lds_bytes<NUP, float>(S.nq);
-        hipLaunchKernelGGL((k_sm_moves<NUP, float>), dim3(grid), dim3(64), lds, st, S, xin, xout,
-                           irn_in, irn_out, sinv, m0, m1, T);
-    } else {
-        const size_t lds = mv_lds_bytes<NUP, double>(S.nq);
-        hipLaunchKernelGGL((k_sm_moves<NUP, double>), dim3(grid), dim3(64), lds, st, S, xin,
-                           xout, irn_in, irn_out, sinv, m0, m1, T);
-    }
-    return hipGetLastError();
+    if (S.nq > 64) return hipErrorInvalidValue;  // four projection chunks of 32
+    const bool one = mv_chunks(S.nq) == 1;
+    if (S.sinv_f32)
+        return one ? sm_moves_tq<NUP, float, 1>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st)
+                   : sm_moves_tq<NUP, float, 4>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
+    return one ? sm_moves_tq<NUP, double, 1>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st)
+               : sm_moves_tq<NUP, double, 4>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
 }
 
 template <int NUP>
