@@ -329,7 +329,7 @@ def main():
     sm = a.workload != "allparticle"
     f = flops_per_walker_step(a.workload, part="moves" if sm else "step")
     achieved = f * main_run["pop_mean"] / (main_run["kernel_ms"] * 1e-3) / 1e12
-    kname = {"mixed": "k_sm_moves<33,float>", "fp64": "k_sm_moves<33,double>",
+    kname = {"mixed": "k_sm_moves32<33,1>", "fp64": "k_sm_moves64<33,1>",
              "allparticle": "k_step<33>"}[a.workload]
     roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                 frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic_from_profiles(a.workload),

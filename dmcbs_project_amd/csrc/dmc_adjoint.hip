@@ -57,7 +57,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
     const double *xs = sm + L.xs, *sinv = sm + L.sinv, *qc = sm + L.qc;
     const double2 *ph = reinterpret_cast<const double2 *>(sm + L.work);
-    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
+    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.qm);
     const int r = T.r, c = T.c;
 
     // Jastrow gradient of every particle (the pair loop of getphi, coshpot.f90:165-202)

@@ -16,11 +16,11 @@ namespace dmc {
 // kernels
 
 template <int NUP>
-__global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict__ x, int nw,
+__global__ __launch_bounds__(64, 2) void k_eval(DevSys S, const double *__restrict__ x, int nw,
                                              double *psil, int *is, double *dpsi,
                                              double *d2psi, double *v, char *sinv) {
     extern __shared__ double sm[];
-    const Layout<NUP> L(S.nq);
+    const Layout<NUP> L(S.nq, true);
     const int w = blockIdx.x, tid = threadIdx.x;
     if (w >= nw) return;
     constexpr int M = 6 * NUP;
@@ -28,9 +28,10 @@ __global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict_
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
     wave_sync();
     const Tile<NUP> T(tid);
+    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
     if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_bytes(S), tid);
-    grad_lap<NUP>(S, sm, L, T, D, tid);
+    grad_lap<NUP>(S, sm, L, T, D, J, tid);
     for (int t = tid; t < M; t += 64) dpsi[size_t(w) * M + t] = sm[L.dp + t];
     if (tid == 0) {
         psil[w] = sm[L.scal];
@@ -44,10 +45,10 @@ __global__ __launch_bounds__(64) void k_eval(DevSys S, const double *__restrict_
 // of state A, written to the same slot of state B, followed by the branch draw
 // (step.f90:302-305): B.irn holds the state after that draw and T.count[w] = iwt.
 template <int NUP>
-__global__ __launch_bounds__(64) void k_step(DevSys S, DevState A, DevState B, DevStep T,
+__global__ __launch_bounds__(64, 2) void k_step(DevSys S, DevState A, DevState B, DevStep T,
                                              int opcomp) {
     extern __shared__ double sm[];
-    const Layout<NUP> L(S.nq);
+    const Layout<NUP> L(S.nq, true);
     const int w = blockIdx.x, tid = threadIdx.x;
     if (w >= T.nt[0]) return;
     constexpr int M = 6 * NUP;                // 3 * npart coordinates
@@ -108,9 +109,9 @@ __device__ __forceinline__ void gauss_pair(const DevSys &S, uint64_t s0, int m, 
 // acceptance) of the state kept.  Walker w of A goes to slot w of B, accepted or not; there
 // is no branching.
 template <int NUP>
-__global__ __launch_bounds__(64) void k_vmc(DevSys S, DevState A, DevState B, DevStep T) {
+__global__ __launch_bounds__(64, 2) void k_vmc(DevSys S, DevState A, DevState B, DevStep T) {
     extern __shared__ double sm[];
-    const Layout<NUP> L(S.nq);
+    const Layout<NUP> L(S.nq, true);
     const int w = blockIdx.x, tid = threadIdx.x;
     if (w >= T.nt[0]) return;
     constexpr int M = 6 * NUP;
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(64) void k_vmc(DevSys S, DevState A, DevState B, De
         gn[t] = acc ? sm[L.dp + t] : go[t];
     }
     // the six estimators of the kept state (addval: val*wt, val^2*wt, wt), in LDS first
-    double *est = sm + L.work;
+    double *est = sm + L.est;
     if (tid < 42) est[tid] = 0.0;
     wave_sync();
     if (tid == 0) {
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(64) void k_ratio(DevSys S, const double *__restrict
     const int i = ip[w];
     const double2 *ph = reinterpret_cast<const double2 *>(sm + L.work);
     double2 *phn = reinterpret_cast<double2 *>(sm + L.xn + 4);
-    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
+    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.qm);
     if (tid == 0) phases_of(S, qm, xn, phn, 1);
     wave_sync();
     const double *qc = sm + L.qc, *sinv = sm + L.sinv, *xs = sm + L.xs;
@@ -261,7 +262,7 @@ bool nup_supported(int nup) { return nup == 1 || nup == 7 || nup == 19 || nup ==
 int launch_eval(int nup, const DevSys &S, const double *x, int nw, double *psil, int *is,
                 double *dpsi, double *d2psi, double *v, char *sinv, hipStream_t st) {
     if (nw <= 0) return hipSuccess;
-    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_eval<N_>, dim3(nw), dim3(64), lds_bytes<N_>(S.nq), st,
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_eval<N_>, dim3(nw), dim3(64), lds_bytes<N_>(S.nq, true), st,
                                          S, x, nw, psil, is, dpsi, d2psi, v, sinv));
     return hipGetLastError();
 }
@@ -277,7 +278,7 @@ int launch_ratio(int nup, const DevSys &S, const double *x, const int *ip, const
 int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
                 const DevStep &T, int opcomp, int grid, hipStream_t st) {
     if (grid <= 0) return hipSuccess;
-    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_step<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq), st,
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_step<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq, true), st,
                                          S, A, B, T, opcomp));
     return hipGetLastError();
 }
@@ -285,17 +286,17 @@ int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
 int launch_vmc(int nup, const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
                int grid, hipStream_t st) {
     if (grid <= 0) return hipSuccess;
-    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_vmc<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq), st,
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_vmc<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq, true), st,
                                          S, A, B, T));
     return hipGetLastError();
 }
 
 size_t allpart_lds_bytes(int nup, int nq) {
     switch (nup) {
-        case 1: return lds_bytes<1>(nq);
-        case 7: return lds_bytes<7>(nq);
-        case 19: return lds_bytes<19>(nq);
-        case 33: return lds_bytes<33>(nq);
+        case 1: return lds_bytes<1>(nq, true);
+        case 7: return lds_bytes<7>(nq, true);
+        case 19: return lds_bytes<19>(nq, true);
+        case 33: return lds_bytes<33>(nq, true);
     }
     return 0;
 }

@@ -109,10 +109,10 @@ size_t batch_wave_lds(const BatchHost &h, BatchPhase ph) {
             }
         } else {
             switch (b.nup) {
-                case 1: v = lds_bytes<1>(nq); break;
-                case 7: v = lds_bytes<7>(nq); break;
-                case 19: v = lds_bytes<19>(nq); break;
-                case 33: v = lds_bytes<33>(nq); break;
+                case 1: v = lds_bytes<1>(nq, true); break;
+                case 7: v = lds_bytes<7>(nq, true); break;
+                case 19: v = lds_bytes<19>(nq, true); break;
+                case 33: v = lds_bytes<33>(nq, true); break;
             }
         }
         m = v > m ? v : m;

@@ -554,7 +554,7 @@ __device__ __forceinline__ void sm_moves_walker_any(const DevSys &S, double *sm,
 template <int NUP>
 __device__ __forceinline__ void sm_refresh_walker(const DevSys &S, double *sm, int w, int tid,
                                                   const double *__restrict__ x, char *sinv_g) {
-    const Layout<NUP> L(S.nq);
+    const Layout<NUP> L(S.nq, true);
     constexpr int M = 6 * NUP;
     load_q<NUP>(S, sm, L, tid);
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
@@ -571,15 +571,16 @@ template <int NUP>
 __device__ __forceinline__ void sm_end_walker(const DevSys &S, double *sm, int w, int tid,
                                               const DevState &A, const DevState &B,
                                               const DevStep &T, int opcomp) {
-    const Layout<NUP> L(S.nq);
+    const Layout<NUP> L(S.nq, true);
     constexpr int M = 6 * NUP;
     load_q<NUP>(S, sm, L, tid);
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = B.x[size_t(w) * M + t];
     wave_sync();
     const Tile<NUP> Tl(tid);
+    const JSums J = jastrow_terms<NUP>(S, sm, L, Tl, tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid);
     store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_bytes(S), tid);
-    grad_lap<NUP>(S, sm, L, Tl, D, tid);
+    grad_lap<NUP>(S, sm, L, Tl, D, J, tid);
     finish_walker<NUP>(S, sm, L, A, B, T, w, tid, B.irn[w], opcomp);
 }
 

@@ -12,8 +12,8 @@
 // column a.
 //
 // Three kernels, so that each gets the registers it needs:
-//  * k_sm_moves: moves m0..m1-1 of one walker per 64-lane workgroup, from the inverse the
-//    walker carries; lanes run over the n opposite-spin partners of the moving particle;
+//  * k_sm_moves32/64: moves m0..m1-1 of one walker per 64-lane workgroup, from the inverse
+//    the walker carries (FP32 or FP64);
 //  * k_sm_refresh: FP64 rebuild of the inverse (every krefresh moves inside a sweep);
 //  * k_sm_end: at the end of the sweep the FP64 hpsi of the final positions gives the
 //    local energy, weight, estimators and branch draw exactly as the all-particle step, and
@@ -28,16 +28,30 @@
 namespace dmc {
 
 // Kernels of one system: walker w = workgroup index, one 64-lane wave per walker.
-template <int NUP, typename R, int NQC>
-__global__ __launch_bounds__(64) void k_sm_moves(DevSys S, const double *__restrict__ xin,
-                                                 double *xout, const uint64_t *irn_in,
-                                                 uint64_t *irn_out, char *sinv_g, int m0,
-                                                 int m1, DevStep T) {
+// FP32 moves: the compiler's own register budget (139 VGPRs at N = 66, three waves per SIMD)
+// times best; FP64 moves: two waves per SIMD asked for (254 VGPRs, no spills), 8.2 ms
+// against 12.6 ms at the one wave per SIMD the compiler picks by itself.
+template <int NUP, int NQC>
+__global__ __launch_bounds__(64) void k_sm_moves32(DevSys S, const double *__restrict__ xin,
+                                                   double *xout, const uint64_t *irn_in,
+                                                   uint64_t *irn_out, char *sinv_g, int m0,
+                                                   int m1, DevStep T) {
     extern __shared__ double sm[];
     const int w = blockIdx.x;
     if (w >= T.nt[0]) return;
-    sm_moves_walker<NUP, R, NQC>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g, m0,
-                                 m1, T);
+    sm_moves_walker<NUP, float, NQC>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g,
+                                     m0, m1, T);
+}
+template <int NUP, int NQC>
+__global__ __launch_bounds__(64, 2) void k_sm_moves64(DevSys S, const double *__restrict__ xin,
+                                                      double *xout, const uint64_t *irn_in,
+                                                      uint64_t *irn_out, char *sinv_g, int m0,
+                                                      int m1, DevStep T) {
+    extern __shared__ double sm[];
+    const int w = blockIdx.x;
+    if (w >= T.nt[0]) return;
+    sm_moves_walker<NUP, double, NQC>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g,
+                                      m0, m1, T);
 }
 
 // FP64 rebuild of the inverse of positions x (every krefresh moves)
@@ -52,9 +66,11 @@ __global__ __launch_bounds__(64) void k_sm_refresh(DevSys S, const double *__res
 
 // end of the sweep: FP64 hpsi of the final positions B.x, the fresh inverse into B.sinv,
 // weight against the energy of A, estimators and branch draw (state after the moves in
-// B.irn)
+// B.irn).  Two waves per SIMD asked for: the compiler spills some of the gradient pass to
+// scratch, which costs less than the latency a single wave per SIMD leaves exposed (at N=66
+// the kernel took 3.0 ms at one wave per SIMD, 2.0 ms at two).
 template <int NUP>
-__global__ __launch_bounds__(64) void k_sm_end(DevSys S, DevState A, DevState B, DevStep T,
+__global__ __launch_bounds__(64, 2) void k_sm_end(DevSys S, DevState A, DevState B, DevStep T,
                                                int opcomp) {
     extern __shared__ double sm[];
     const int w = blockIdx.x;
@@ -67,8 +83,12 @@ int sm_moves_tq(const DevSys &S, const double *xin, double *xout, const uint64_t
                 uint64_t *irn_out, char *sinv, int m0, int m1, const DevStep &T, int grid,
                 hipStream_t st) {
     const size_t lds = mv_lds_bytes<NUP, R>(S.nq);
-    hipLaunchKernelGGL((k_sm_moves<NUP, R, NQC>), dim3(grid), dim3(64), lds, st, S, xin, xout,
-                       irn_in, irn_out, sinv, m0, m1, T);
+    if constexpr (sizeof(R) == 8)
+        hipLaunchKernelGGL((k_sm_moves64<NUP, NQC>), dim3(grid), dim3(64), lds, st, S, xin, xout,
+                           irn_in, irn_out, sinv, m0, m1, T);
+    else
+        hipLaunchKernelGGL((k_sm_moves32<NUP, NQC>), dim3(grid), dim3(64), lds, st, S, xin, xout,
+                           irn_in, irn_out, sinv, m0, m1, T);
     return hipGetLastError();
 }
 
@@ -88,7 +108,7 @@ int sm_moves_t(const DevSys &S, const double *xin, double *xout, const uint64_t 
 template <int NUP>
 int sm_refresh_t(const DevSys &S, const double *x, char *sinv, const DevStep &T, int grid,
                  hipStream_t st) {
-    hipLaunchKernelGGL(k_sm_refresh<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq), st, S, x,
+    hipLaunchKernelGGL(k_sm_refresh<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq, true), st, S, x,
                        sinv, T);
     return hipGetLastError();
 }
@@ -96,7 +116,7 @@ int sm_refresh_t(const DevSys &S, const double *x, char *sinv, const DevStep &T,
 template <int NUP>
 int sm_end_t(const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
              int opcomp, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_sm_end<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq), st, S, A, B, T,
+    hipLaunchKernelGGL(k_sm_end<NUP>, dim3(grid), dim3(64), lds_bytes<NUP>(S.nq, true), st, S, A, B, T,
                        opcomp);
     return hipGetLastError();
 }

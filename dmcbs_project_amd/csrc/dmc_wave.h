@@ -99,28 +99,25 @@ __device__ __forceinline__ double wave_sum(double v) {
                   __builtin_amdgcn_readlane(int(b >> 32), 63));
 }
 
-// exp(i m theta) from the precomputed powers, |m| <= 4
+// exp(i m theta) from the precomputed powers, |m| <= 4.  m is wave-uniform (the same q
+// for every lane): scalar branches pick the power, no per-lane selects.
 __device__ __forceinline__ void cpow(const double (&pc)[5], const double (&ps)[5], int m,
                                      double &c, double &s) {
-    // selects kept apart by scheduling barriers so that they are not folded into an
-    // indexed load from a private (scratch) array
+    m = __builtin_amdgcn_readfirstlane(m);
     const int am = m < 0 ? -m : m;
-    c = pc[0];
-    s = ps[0];
-#pragma unroll
-    for (int k = 1; k < 5; ++k) {
-        const bool t = am == k;
-        c = t ? pc[k] : c;
-        s = t ? ps[k] : s;
-        __builtin_amdgcn_sched_barrier(0);
+    switch (am) {
+        case 0: c = 1.0; s = 0.0; break;
+        case 1: c = pc[1]; s = ps[1]; break;
+        case 2: c = pc[2]; s = ps[2]; break;
+        case 3: c = pc[3]; s = ps[3]; break;
+        default: c = pc[4]; s = ps[4]; break;
     }
     s = m < 0 ? -s : s;
 }
 
-// exp(i q.r) for all q of one position
-__device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const double *pos,
-                                          double2 *out, int stride) {
-    double pc[3][5], ps[3][5];
+// exp(i m 2 pi x_d / L), m = 0..4, of the three coordinates of one position
+__device__ __forceinline__ void axis_powers(const DevSys &S, const double *pos, double (&pc)[3][5],
+                                            double (&ps)[3][5]) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double s1, c1;
@@ -133,6 +130,13 @@ __device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const
             ps[d][m] = ps[d][m - 1] * c1 + pc[d][m - 1] * s1;
         }
     }
+}
+
+// exp(i q.r) for all q of one position (q-vectors wave-uniform)
+__device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const double *pos,
+                                          double2 *out, int stride) {
+    double pc[3][5], ps[3][5];
+    axis_powers(S, pos, pc, ps);
     for (int q = 0; q < S.nq; ++q) {
         const int4 m = qm[q];
         double cx, sx, cy, sy, cz, sz;
@@ -144,38 +148,89 @@ __device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const
     }
 }
 
+// exp(i q.r) of every particle into ph[q * NP + p]: lanes over the first 64 particles
+// through their per-axis powers; particles beyond the 64th (N > 64) one (particle, q) pair
+// per lane with a direct sincos of q.r, so that the wave makes one pass of the slow path
+template <int NUP>
+__device__ __forceinline__ void phases_all(const DevSys &S, const double *xs, const int4 *qm,
+                                           double2 *ph, int tid) {
+    constexpr int NP = 2 * NUP;
+    if (tid < NP) phases_of(S, qm, xs + 3 * tid, ph + tid, NP);
+    if constexpr (NP > 64) {
+        const int nq = S.nq;
+        for (int t = tid; t < (NP - 64) * nq; t += 64) {
+            const int e = t / nq, q = t - e * nq, p = 64 + e;
+            const int4 m = qm[q];
+            const double *pos = xs + 3 * p;
+            const double th = S.twopi_l * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            ph[q * NP + p] = make_double2(cs, sn);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // LDS layout of one walker (offsets in doubles)
+//
+// Full layout (single-particle ratio, parameter adjoint): the phases of every particle and
+// the inverse stay in LDS after build_inverse.  Lean layout (hpsi, step and refresh
+// kernels): phases, elimination scratch, inverse and partial sums share one area in turn,
+// and the gradient pass rebuilds the phases once the inverse is in registers, so that two
+// walkers fit where one did.
 
 template <int NUP>
 struct Layout {
-    static constexpr int TR = (NUP + 7) / 8, R8 = 8 * TR, SP = R8 + 1, NP = 2 * NUP;
-    int xs, dp, xn, qc, prow, mcol, ints, work, sinv, scal, total;
-    __host__ __device__ explicit Layout(int nq) {
+    static constexpr int TR = (NUP + 7) / 8, R8 = 8 * TR, NP = 2 * NUP;
+    // row stride of the inverse and of the partial sums: = 8 (mod 16), so that the 8 x 8
+    // lane grid reading a tile (rows c + 8b, columns r + 8a) hits distinct banks
+    static constexpr int SP = (NUP + 7) / 16 * 16 + 8;
+    int xs, qc, qm, dp, xn, prow, mcol, ints, work, sinv, est, scal, total;
+    bool lean;
+    __host__ __device__ explicit Layout(int nq, bool lean_ = false) : lean(lean_) {
+        const int phs = 2 * NP * nq;          // phases, [nq][NP] complex
+        const int inv = NUP * SP;             // Sinv(j, i) at [j * SP + i]
+        const int part = 6 * 8 * SP;          // partial sums of one spin, [6][8][SP]
         xs = 0;
-        dp = xs + 3 * NP;
-        xn = dp + 3 * NP;           // one extra position (4) and its phases (2 nq)
-        qc = xn + 4 + 2 * nq;       // per q: wq, gq.x, gq.y, gq.z, lq
-        prow = qc + 5 * nq;
-        mcol = prow + R8;
-        ints = mcol + R8;           // perm[64], pinv[64], qm[nq] (int4)
-        work = ints + 64 + 2 * nq;
-        const int phs = 2 * NP * nq;
-        sinv = work + phs;
-        const int w1 = phs + R8 * SP, w2 = 96 * NUP;
-        scal = work + (w1 > w2 ? w1 : w2);
+        qc = xs + 3 * NP;                     // per q: w, -w q (3), -w q^2
+        qm = qc + 5 * nq;                     // integer q vectors (int4)
+        if (!lean) {
+            dp = qm + 2 * nq;
+            xn = dp + 3 * NP;                 // one extra position (4) and its phases (2 nq)
+            prow = xn + 4 + 2 * nq;
+            mcol = prow + R8;
+            ints = mcol + R8;                 // perm[64], pinv[64]
+            work = ints + 64;
+            sinv = work + phs;
+            est = work;
+            scal = work + (phs + inv > part ? phs + inv : part);
+        } else {
+            work = qm + 2 * nq;
+            sinv = work;
+            ints = sinv + inv;                // live until the inverse is scattered
+            prow = ints + 64;
+            mcol = prow + R8;
+            dp = work;                        // written once the partial sums are read
+            est = dp + 3 * NP;
+            xn = -1;                          // no extra position
+            int w = phs;
+            if (mcol + R8 - work > w) w = mcol + R8 - work;
+            if (part > w) w = part;
+            if (3 * NP + 42 > w) w = 3 * NP + 42;
+            scal = work + w;
+        }
         total = scal + 16;
     }
 };
 
 template <int NUP>
-size_t lds_bytes(int nq) { return sizeof(double) * Layout<NUP>(nq).total; }
+size_t lds_bytes(int nq, bool lean = false) { return sizeof(double) * Layout<NUP>(nq, lean).total; }
 
 // Load q coefficients into LDS (all lanes)
 template <int NUP>
 __device__ __forceinline__ void load_q(const DevSys &S, double *sm, const Layout<NUP> &L,
                                        int tid) {
-    int4 *qm = reinterpret_cast<int4 *>(sm + L.ints + 64);
+    int4 *qm = reinterpret_cast<int4 *>(sm + L.qm);
     const double tp = S.twopi_l;
     for (int q = tid; q < S.nq; q += 64) {
         const int4 m = S.qm[q];
@@ -230,7 +285,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     const int r = T.r, c = T.c;
     const double *xs = sm + L.xs;
     double2 *ph = reinterpret_cast<double2 *>(sm + L.work);
-    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.ints + 64);
+    const int4 *qm = reinterpret_cast<const int4 *>(sm + L.qm);
     int *perm = reinterpret_cast<int *>(sm + L.ints);
     int *pinv = perm + 64;
     const double *qc = sm + L.qc;
@@ -239,7 +294,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
 
     // Phase A: plane-wave phases of every particle
     // q-major: ph[q * NP + p], so that lanes over particles read consecutive words
-    for (int p = tid; p < NP; p += 64) phases_of(S, qm, xs + 3 * p, ph + p, NP);
+    phases_all<NUP>(S, xs, qm, ph, tid);
     wave_sync();
 
     // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij)
@@ -272,6 +327,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
 #pragma unroll
             for (int b = 0; b < TR; ++b) Sm[a][b] += zi[a].x * zj[b].x + zi[a].y * zj[b].y;
     }
+    wave_sync();  // the lean layout puts the elimination scratch over the phases
 
     // Phase C: Gauss-Jordan inversion with partial pivoting, rows never moved
     double *prow = sm + L.prow, *mcol = sm + L.mcol;
@@ -282,17 +338,27 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     double prod = 1.0;
     int expo = 0, sgn = 1;
     bool singular = false;
-#pragma unroll
+    // One pivot per iteration, the loop kept rolled: a compact body whose registers are
+    // those of one step.  Tile column bk of the pivot column and tile row ap of the pivot
+    // row are wave-uniform, picked by scalar branches.
+#pragma unroll 1
     for (int k = 0; k < NUP; ++k) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int bk = k >> 3, ck = k & 7;
+        double colk[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) colk[a] = Sm[a][0];
+#pragma unroll
+        for (int b = 1; b < TR; ++b)
+            if (b == bk) {
+#pragma unroll
+                for (int a = 0; a < TR; ++a) colk[a] = Sm[a][b];
+            }
         double best = -1.0;
         int brow = 1 << 20;
         if (c == ck) {
 #pragma unroll
             for (int a = 0; a < TR; ++a) {
-                const double v = fabs(Sm[a][bk]);
+                const double v = fabs(colk[a]);
                 if (!((used >> a) & 1u) && v > best) {
                     best = v;
                     brow = r + 8 * a;
@@ -309,18 +375,23 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
             }
         }
         const int p = __builtin_amdgcn_readlane(brow, ck);
+        const int ap = p >> 3;
         if (r == (p & 7)) {
-            const int ap = p >> 3;
+            double rowp[TR];
 #pragma unroll
-            for (int a = 0; a < TR; ++a)
+            for (int b = 0; b < TR; ++b) rowp[b] = Sm[0][b];
+#pragma unroll
+            for (int a = 1; a < TR; ++a)
                 if (a == ap) {
 #pragma unroll
-                    for (int b = 0; b < TR; ++b) prow[c + 8 * b] = Sm[a][b];
+                    for (int b = 0; b < TR; ++b) rowp[b] = Sm[a][b];
                 }
+#pragma unroll
+            for (int b = 0; b < TR; ++b) prow[c + 8 * b] = rowp[b];
         }
         if (c == ck) {
 #pragma unroll
-            for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][bk];
+            for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = colk[a];
         }
         if (tid == 0) {
             perm[k] = p;
@@ -342,18 +413,30 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
         for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a] * rp;
 #pragma unroll
         for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
+        // elimination of every element, then the pivot column (lanes c == ck, tile column
+        // bk) and the pivot row (lanes r == p & 7, tile row ap) overwritten: bk and ap are
+        // wave-uniform, so only the selects of one tile column and one tile row remain
 #pragma unroll
-        for (int a = 0; a < TR; ++a) {
-            const bool isp = (r + 8 * a) == p;
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
-            for (int b = 0; b < TR; ++b) {
-                const bool jk = (b == bk) && (c == ck);
-                const double upd = jk ? -f[a] : Sm[a][b] - f[a] * pv[b];
-                const double prw = jk ? rp : pv[b] * rp;
-                Sm[a][b] = isp ? prw : upd;
+            for (int b = 0; b < TR; ++b) Sm[a][b] -= f[a] * pv[b];
+        const bool colk_lane = c == ck, prow_lane = r == (p & 7);
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+            if (b == bk) {
+#pragma unroll
+                for (int a = 0; a < TR; ++a) Sm[a][b] = colk_lane ? -f[a] : Sm[a][b];
             }
-            if (isp) used |= 1u << a;
-        }
+#pragma unroll
+        for (int a = 0; a < TR; ++a)
+            if (a == ap) {
+#pragma unroll
+                for (int b = 0; b < TR; ++b) {
+                    const double prw = (b == bk && colk_lane) ? rp : pv[b] * rp;
+                    Sm[a][b] = prow_lane ? prw : Sm[a][b];
+                }
+                if (prow_lane) used |= 1u << a;
+            }
         wave_sync();
     }
 
@@ -383,20 +466,115 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     return DetInfo{logdet, sgn, singular};
 }
 
+// Jastrow and potential terms of the pair loop of getphi (coshpot.f90:165-202), opposite
+// spins: per particle, lane p < NUP holds the Jastrow gradient of up particle p (j[0]) and
+// down particle p (j[1]); the sums U, V and d2U are wave-uniform.  Runs before the pairing
+// matrix is built (the lean layout's work area is free then), so that its partial sums are
+// never live together with the inverse.
+struct JSums {
+    double j[2][3];
+    double U, V, D2U;
+};
+
+template <int NUP>
+__device__ JSums jastrow_terms(const DevSys &S, double *sm, const Layout<NUP> &L,
+                               const Tile<NUP> &T, int tid) {
+    constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP;
+    constexpr int PS = 3 * 8 * SP;
+    const double *xs = sm + L.xs;
+    double *part = sm + L.work;
+    const int r = T.r, c = T.c;
+    double U = 0.0, V = 0.0, D2U = 0.0;
+    double pu[TR][3], pdn[TR][3];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pu[a][d] = pdn[a][d] = 0.0;
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TR; ++b) {
+            if (!(T.rv[a] && T.cv[b])) continue;
+            const double *xi = xs + 3 * T.ri[a], *xj = xs + 3 * (NUP + T.cj[b]);
+            // minimum image by round-to-nearest-even: at an exact tie both images have the
+            // same distance and every pair term has zero slope there (r = L/2)
+            double dx[3], r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                dx[d] = xi[d] - xj[d];
+                dx[d] -= S.el * rint(dx[d] * S.eli);
+                r2 += dx[d] * dx[d];
+            }
+            const double rj = fmax(1.0e-14, sqrt(r2));
+            if (rj < S.range) {
+                double w0, w1, w2, w3;
+                const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+                const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
+                V += jv.x;
+                U += jv.y;
+                const double irj = 1.0 / rj;
+                const double t = jv.z * irj * S.invsqrtm;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    pu[a][d] += dx[d] * t;
+                    pdn[b][d] -= dx[d] * t;
+                }
+                D2U += (jv.w + 2.0 * jv.z * irj) * S.invmu;
+            }
+            // a scheduling fence per pair: the table loads of one pair in flight at a time
+            // (unfenced, the straight-line loop issues every pair's loads at once)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+    for (int a = 0; a < TR; ++a) {
+        if (T.rv[a]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + T.ri[a]] = pu[a][d];
+        }
+        if (T.cv[a]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + T.cj[a]] = pdn[a][d];
+        }
+    }
+    wave_sync();
+    JSums J;
+    const int pl = tid < NUP ? tid : 0;
+#pragma unroll
+    for (int spin = 0; spin < 2; ++spin)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
+            J.j[spin][d] = v;
+        }
+    J.U = wave_sum(U);
+    J.V = wave_sum(V);
+    J.D2U = wave_sum(D2U);
+    wave_sync();
+    return J;
+}
+
 // Phases E-G: gradients and laplacian of log det through the inverse, Jastrow and
-// potential, and the hpsi combination (wavefunction.f90:171-201).  Results in LDS:
-// scal[0..4] = psil, d2psi, v, is, log|det|; dp = dpsi.  Consumes the phases and the
-// inverse (their area is reused for the partial sums).
+// potential, and the hpsi combination (wavefunction.f90:171-201), on the lean layout.
+// Results in LDS: scal[0..4] = psil, d2psi, v, is, log|det|; dp = dpsi.  The inverse goes
+// into the lane tile first; the Jastrow gradients are summed per particle through the work
+// area, then the phases are rebuilt there for the plane-wave contraction, so that the lanes
+// hold one set of partial sums at a time.  Partial sums travel as [quantity][lane row or
+// column][particle] (row stride SP): the stores of the lane grid and the loads of lanes over
+// particles both hit distinct banks.
 template <int NUP>
 __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, const Tile<NUP> &T,
-                         const DetInfo &D, int tid) {
+                         const DetInfo &D, const JSums &J, int tid) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
+    constexpr int PS = 3 * 8 * SP;  // partial sums of one spin, [3][8][SP]
     const int nq = S.nq;
     const int r = T.r, c = T.c;
     const double *xs = sm + L.xs;
     const double2 *ph = reinterpret_cast<const double2 *>(sm + L.work);
     const double *qc = sm + L.qc;
     const double *sinv = sm + L.sinv;
+    double *part = sm + L.work;
     const int *ri = T.ri, *cj = T.cj;
     const bool *rv = T.rv, *cv = T.cv;
     const bool singular = D.singular;
@@ -408,53 +586,82 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
 #pragma unroll
         for (int b = 0; b < TR; ++b)
             sv[a][b] = (rv[a] && cv[b] && !singular) ? sinv[cj[b] * SP + ri[a]] : 0.0;
-    double gu[TR][3], gd[TR][3], ju[TR][3], jd[TR][3];
+    wave_sync();  // the inverse is in registers: the work area is free
+
+    // radial part of the pairing functions, summed per particle through the work area
+    const int pl = tid < NUP ? tid : 0;
+    double lap = 0.0;
+    double gs[2][3];
+    {
+        double pu[TR][3], pdn[TR][3];
 #pragma unroll
-    for (int a = 0; a < TR; ++a)
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) gu[a][d] = gd[a][d] = ju[a][d] = jd[a][d] = 0.0;
-    double lap = 0.0, U = 0.0, V = 0.0, D2U = 0.0;
+            for (int d = 0; d < 3; ++d) pu[a][d] = pdn[a][d] = 0.0;
 #pragma unroll
-    for (int a = 0; a < TR; ++a)
+        for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int b = 0; b < TR; ++b) {
-            if (!(rv[a] && cv[b])) continue;
-            const double *xi = xs + 3 * ri[a], *xj = xs + 3 * (NUP + cj[b]);
-            double dx[3], r2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                dx[d] = wrapd(xi[d] - xj[d], S);
-                r2 += dx[d] * dx[d];
-            }
-            const double rr = sqrt(r2);
-            if (!S.nobeta && rr <= S.rangeb) {
-                double w0, w1, w2, w3;
-                const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
-                const double4 bv = lookup4(S.bt, idx, w0, w1, w2, w3);
-                const double t = sv[a][b] * bv.y / rr;
+            for (int b = 0; b < TR; ++b) {
+                if (!(rv[a] && cv[b])) continue;
+                const double *xi = xs + 3 * ri[a], *xj = xs + 3 * (NUP + cj[b]);
+                double dx[3], r2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    gu[a][d] += t * dx[d];
-                    gd[b][d] -= t * dx[d];
+                    dx[d] = xi[d] - xj[d];
+                    dx[d] -= S.el * rint(dx[d] * S.eli);
+                    r2 += dx[d] * dx[d];
                 }
-                lap += sv[a][b] * bv.z;
-            }
-            const double rj = fmax(1.0e-14, rr);
-            if (rj < S.range) {
-                double w0, w1, w2, w3;
-                const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
-                const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
-                V += jv.x;
-                U += jv.y;
-                const double t = jv.z / (rj * S.sqrtm);
+                const double rr = sqrt(r2);
+                if (!S.nobeta && rr <= S.rangeb) {
+                    double w0, w1, w2, w3;
+                    const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
+                    const double4 bv = lookup4(S.bt, idx, w0, w1, w2, w3);
+                    const double t = sv[a][b] * bv.y * (1.0 / rr);
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    ju[a][d] += dx[d] * t;
-                    jd[b][d] -= dx[d] * t;
+                    for (int d = 0; d < 3; ++d) {
+                        pu[a][d] += t * dx[d];
+                        pdn[b][d] -= t * dx[d];
+                    }
+                    lap += sv[a][b] * bv.z;
                 }
-                D2U += (jv.w + 2.0 * jv.z / rj) * S.invmu;
+                __builtin_amdgcn_sched_barrier(0);  // one pair's table loads in flight
+            }
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            if (rv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = pu[a][d];
+            }
+            if (cv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = pdn[a][d];
             }
         }
+    }
+    wave_sync();
+#pragma unroll
+    for (int spin = 0; spin < 2; ++spin)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
+            gs[spin][d] = v;
+        }
+    wave_sync();
+    phases_all<NUP>(S, xs, reinterpret_cast<const int4 *>(sm + L.qm),
+                    reinterpret_cast<double2 *>(sm + L.work), tid);
+    wave_sync();
+
+    double gu[TR][3], gd[TR][3];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gu[a][d] = gd[a][d] = 0.0;
+    // plane-wave part: Sum_j Sinv(j, i) sin(q.r_ij) = s_i X_i - c_i Y_i with X = C^T Sinv
+    // for the up particles, and the same with Z, W for the down particles, in two passes
+    // over q (one set of accumulators live at a time)
+#pragma unroll 1
     for (int q = 0; q < nq; ++q) {
         const double *cq = qc + 5 * q;
         double2 zi[TR], zj[TR];
@@ -463,87 +670,94 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
             zi[a] = ph[q * NP + ri[a]];
             zj[a] = ph[q * NP + NUP + cj[a]];
         }
-        double X[TR], Y[TR], Z[TR], W[TR];
-#pragma unroll
-        for (int a = 0; a < TR; ++a) {
-            X[a] = Y[a] = Z[a] = W[a] = 0.0;
-        }
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < TR; ++b) {
-                X[a] += sv[a][b] * zj[b].x;
-                Y[a] += sv[a][b] * zj[b].y;
-                Z[b] += sv[a][b] * zi[a].y;
-                W[b] += sv[a][b] * zi[a].x;
-            }
         double ls = 0.0;
 #pragma unroll
         for (int a = 0; a < TR; ++a) {
-            const double t = zi[a].y * X[a] - zi[a].x * Y[a];
+            double X = 0.0, Y = 0.0;
+#pragma unroll
+            for (int b = 0; b < TR; ++b) {
+                X += sv[a][b] * zj[b].x;
+                Y += sv[a][b] * zj[b].y;
+            }
+            const double t = zi[a].y * X - zi[a].x * Y;
             gu[a][0] += cq[1] * t;
             gu[a][1] += cq[2] * t;
             gu[a][2] += cq[3] * t;
-            ls += zi[a].x * X[a] + zi[a].y * Y[a];
-            const double u = zj[a].x * Z[a] - zj[a].y * W[a];
-            gd[a][0] -= cq[1] * u;
-            gd[a][1] -= cq[2] * u;
-            gd[a][2] -= cq[3] * u;
+            ls += zi[a].x * X + zi[a].y * Y;
         }
         lap += cq[4] * ls;
     }
-    wave_sync();  // done with ph and sinv: reuse the work area for partial sums
+#pragma unroll 1
+    for (int q = 0; q < nq; ++q) {
+        const double *cq = qc + 5 * q;
+        double2 zi[TR], zj[TR];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            zi[a] = ph[q * NP + ri[a]];
+            zj[a] = ph[q * NP + NUP + cj[a]];
+        }
+#pragma unroll
+        for (int b = 0; b < TR; ++b) {
+            double Z = 0.0, W = 0.0;
+#pragma unroll
+            for (int a = 0; a < TR; ++a) {
+                Z += sv[a][b] * zi[a].y;
+                W += sv[a][b] * zi[a].x;
+            }
+            const double u = zj[b].x * Z - zj[b].y * W;
+            gd[b][0] -= cq[1] * u;
+            gd[b][1] -= cq[2] * u;
+            gd[b][2] -= cq[3] * u;
+        }
+    }
+    wave_sync();  // done with the phases
 
-    // Phase G: reduce per-particle partial sums across the lane grid
-    double *pu = sm + L.work;            // [6][NUP][8]  up particles, over c
-    double *pd = pu + 48 * NUP;          // [6][NUP][8]  down particles, over r
+    // determinant gradients per particle
 #pragma unroll
     for (int a = 0; a < TR; ++a) {
         if (rv[a]) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                pu[(d * NUP + ri[a]) * 8 + c] = gu[a][d];
-                pu[((3 + d) * NUP + ri[a]) * 8 + c] = ju[a][d];
-            }
+            for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = gu[a][d];
         }
         if (cv[a]) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                pd[(d * NUP + cj[a]) * 8 + r] = gd[a][d];
-                pd[((3 + d) * NUP + cj[a]) * 8 + r] = jd[a][d];
-            }
+            for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = gd[a][d];
         }
     }
     wave_sync();
-    double du2 = 0.0, dudp = 0.0;
-    for (int p = tid; p < NP; p += 64) {
-        const double *src = p < NUP ? pu : pd;
-        const int pp = p < NUP ? p : p - NUP;
+#pragma unroll
+    for (int spin = 0; spin < 2; ++spin)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double g = 0.0, j = 0.0;
+            double g = 0.0;
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                g += src[(d * NUP + pp) * 8 + l];
-                j += src[((3 + d) * NUP + pp) * 8 + l];
+            for (int l = 0; l < 8; ++l) g += part[spin * PS + (d * 8 + l) * SP + pl];
+            gs[spin][d] += g;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    wave_sync();  // dp shares the work area
+    double du2 = 0.0, dudp = 0.0;
+    if (tid < NUP) {
+#pragma unroll
+        for (int spin = 0; spin < 2; ++spin) {
+            const int p = spin * NUP + tid;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double g = singular ? 0.0 : gs[spin][d] * S.invsqrtm, j = J.j[spin][d];
+                sm[L.dp + 3 * p + d] = g + j;
+                du2 += j * j;
+                dudp += j * g;
             }
-            g = singular ? 0.0 : g * S.invsqrtm;
-            sm[L.dp + 3 * p + d] = g + j;
-            du2 += j * j;
-            dudp += j * g;
         }
     }
     lap = wave_sum(lap);
-    U = wave_sum(U);
-    V = wave_sum(V);
-    D2U = wave_sum(D2U);
     du2 = wave_sum(du2);
     dudp = wave_sum(dudp);
     if (tid == 0) {
         const double d2phi = singular ? 0.0 : 2.0 * lap * S.invm;
-        sm[L.scal + 0] = logdet + U;
-        sm[L.scal + 1] = d2phi + D2U + du2 + 2.0 * dudp;
-        sm[L.scal + 2] = V;
+        sm[L.scal + 0] = logdet + J.U;
+        sm[L.scal + 1] = d2phi + J.D2U + du2 + 2.0 * dudp;
+        sm[L.scal + 2] = J.V;
         sm[L.scal + 3] = double(sgn);
         sm[L.scal + 4] = logdet;
     }
@@ -580,8 +794,9 @@ template <int NUP>
 __device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
                                            int tid) {
     const Tile<NUP> T(tid);
+    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
-    grad_lap<NUP>(S, sm, L, T, D, tid);
+    grad_lap<NUP>(S, sm, L, T, D, J, tid);
 }
 
 // Jastrow ratio exp(sum_j u(|x' - r_j|) - u(|r_i - r_j|)) over the opposite-spin partners
@@ -645,7 +860,7 @@ __device__ void finish_walker(const DevSys &S, double *sm, const Layout<NUP> &L,
     }
     // the 42 estimator values are assembled in LDS (the work area is free by now) and
     // stored once, so that no two lanes store to one global address
-    double *est = sm + L.work;
+    double *est = sm + L.est;
     if (tid < 42) est[tid] = 0.0;
     wave_sync();
     if (tid == 0) {
