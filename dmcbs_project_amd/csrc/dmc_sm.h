@@ -35,6 +35,36 @@ __device__ __forceinline__ R rcp_r(R x) {
     else return __builtin_amdgcn_rcpf(x);
 }
 
+// sin and cos of 2 pi t for |t| <= 1/2 (t in turns), FP32: reduction to the nearest quarter
+// turn (|x| <= pi/4), the Taylor polynomials of sin to x^9 and of cos to x^10 (truncation
+// errors 1.8e-9 and 1.1e-10 at pi/4), then the rotation by that quarter turn
+__device__ __forceinline__ void sincos_turns(float t, float &s, float &c) {
+    const float q = rintf(4.0f * t);
+    const float x = fmaf(-0.25f, q, t) * 6.28318530717958648f;
+    const float x2 = x * x;
+    float sp = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+    sp = fmaf(x2, sp, 8.33333333e-3f);
+    sp = fmaf(x2, sp, -1.66666667e-1f);
+    const float sn = fmaf(x * x2, sp, x);
+    float cp = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
+    cp = fmaf(x2, cp, -1.38888889e-3f);
+    cp = fmaf(x2, cp, 4.16666667e-2f);
+    cp = fmaf(x2, cp, -0.5f);
+    const float cs = fmaf(x2, cp, 1.0f);
+    const int iq = int(q) & 3;
+    s = iq == 0 ? sn : iq == 1 ? cs : iq == 2 ? -sn : -cs;
+    c = iq == 0 ? cs : iq == 1 ? -sn : iq == 2 ? -cs : sn;
+}
+
+// sin and cos of 2 pi t for a turn count t given in FP64: FP64 reduction to the nearest
+// turn, then FP64 sincos (R = double) or the FP32 polynomials above
+template <typename R>
+__device__ __forceinline__ void sincos_2pi(double t, R &s, R &c) {
+    t -= rint(t);
+    if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * t, &s, &c);
+    else sincos_turns(float(t), s, c);
+}
+
 // distance of the pair (a, b), the minimum-image difference formed in FP64 and the rest in R
 template <typename R>
 __device__ __forceinline__ R pair_dist(const DevSys &S, const double *xa, const double *xb,
@@ -376,7 +406,12 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                 }
             }
         }
-        const double go[3] = {double(wave_sum(g0)), double(wave_sum(g1)), double(wave_sum(g2))};
+        if constexpr (sizeof(R) == 4) wave_sum2(g0, g1);
+        else {
+            g0 = wave_sum(g0);
+            g1 = wave_sum(g1);
+        }
+        const double go[3] = {double(g0), double(g1), double(wave_sum(g2))};
         // gaussian(3) then the acceptance draw: five ran1 draws per move
         double u[5];
 #pragma unroll
@@ -387,13 +422,8 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         double g3[3];
         {
             R s1, c1, s3, c3;
-            if constexpr (sizeof(R) == 8) {
-                sincos(2.0 * kPi * u[0], &s1, &c1);
-                sincos(2.0 * kPi * u[2], &s3, &c3);
-            } else {
-                sincosf(float(2.0 * kPi * u[0]), &s1, &c1);
-                sincosf(float(2.0 * kPi * u[2]), &s3, &c3);
-            }
+            sincos_2pi<R>(u[0], s1, c1);
+            sincos_2pi<R>(u[2], s3, c3);
             const R r12 = sqrt(R(-2) * log(R(u[1]))), r34 = sqrt(R(-2) * log(R(u[3])));
             g3[0] = double(s1 * r12);
             g3[1] = double(c1 * r12);
@@ -414,14 +444,8 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         for (int c = 0; c < NQC; ++c) {
             const int o = ol + 32 * c;
             const int4 m = cm[c];
-            double turns = (m.x * xn[0] + m.y * xn[1] + m.z * xn[2]) * S.eli;
-            turns -= rint(turns);
             R sn, cs;
-            if constexpr (sizeof(R) == 8) {
-                sincos(2.0 * kPi * turns, &sn, &cs);
-            } else {
-                sincosf(float(2.0 * kPi * turns), &sn, &cs);
-            }
+            sincos_2pi<R>((m.x * xn[0] + m.y * xn[1] + m.z * xn[2]) * S.eli, sn, cs);
             unc[c] = cs;
             uns[c] = sn;
             if (h == 0 && o < no) ub[o] = cw[c] * ((o & 1) ? sn : cs);
@@ -473,9 +497,23 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             have_old = true;
         }
         const double rho = wave_sum(rho_part);  // FP64 sum: the node test uses its sign
-        const double gn[3] = {double(wave_sum(gn0)), double(wave_sum(gn1)), double(wave_sum(gn2))};
-        const double jn[3] = {double(wave_sum(jn0)), double(wave_sum(jn1)), double(wave_sum(jn2))};
-        const double dlogj = double(wave_sum(dU));
+        if constexpr (sizeof(R) == 4) {
+            wave_sum2(gn0, gn1);
+            wave_sum2(gn2, jn0);
+            wave_sum2(jn1, jn2);
+            dU = wave_sum(dU);
+        } else {
+            gn0 = wave_sum(gn0);
+            gn1 = wave_sum(gn1);
+            gn2 = wave_sum(gn2);
+            jn0 = wave_sum(jn0);
+            jn1 = wave_sum(jn1);
+            jn2 = wave_sum(jn2);
+            dU = wave_sum(dU);
+        }
+        const double gn[3] = {double(gn0), double(gn1), double(gn2)};
+        const double jn[3] = {double(jn0), double(jn1), double(jn2)};
+        const double dlogj = double(dU);
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
             const double irho = 1.0 / rho;

@@ -73,12 +73,32 @@ __device__ __forceinline__ int dpp_bcast(int v, int k) {
     return k == 0 ? __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false)   // row_bcast:15
                   : __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
 }
+// the same row exchanges with bound_ctrl set (every source lane of these patterns is in the
+// row), so that the compiler folds each exchange into the add (v_add_f32_dpp)
+__device__ __forceinline__ float row_sum16(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, true));
+    return v;
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v += __int_as_float(dpp_row(__float_as_int(v), k));
+    v = row_sum16(v);
 #pragma unroll
     for (int k = 0; k < 2; ++k) v += __int_as_float(dpp_bcast(__float_as_int(v), k));
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// Sums over the 64 lanes of two values at once: a permlane swap of the half-waves puts the
+// half-wave pair sums of a in lanes 0-31 and of b in lanes 32-63, rows of 16 are summed,
+// and row_bcast:15 joins rows 0+1 (a, lane 31) and rows 2+3 (b, lane 63).
+__device__ __forceinline__ void wave_sum2(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                    false);
+    float v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    v = row_sum16(v);
+    v += __int_as_float(dpp_bcast(__float_as_int(v), 0));
+    a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ double dbl_of(int lo, int hi) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
