@@ -65,6 +65,21 @@ __device__ __forceinline__ void sincos_2pi(double t, R &s, R &c) {
     else sincos_turns(float(t), s, c);
 }
 
+// the value of v in lane l, wave-uniform
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    return dbl_of(__builtin_amdgcn_readlane(int(b & 0xffffffffll), l),
+                  __builtin_amdgcn_readlane(int(b >> 32), l));
+}
+__device__ __forceinline__ uint64_t lane_value(uint64_t v, int l) {
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(v & 0xffffffffull), l);
+    const unsigned hi = __builtin_amdgcn_readlane(unsigned(v >> 32), l);
+    return (uint64_t(hi) << 32) | lo;
+}
+
 // distance of the pair (a, b), the minimum-image difference formed in FP64 and the rest in R
 template <typename R>
 __device__ __forceinline__ R pair_dist(const DevSys &S, const double *xa, const double *xb,
@@ -327,6 +342,17 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
     const R alpha0 = R(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
+    // jump-ahead coefficients of this lane's draws (state after t + 1 draws = ja1[t] s +
+    // jc1[t]): lane l < 2 draws 2l, 2l + 1; lane 2 draw 4; other lanes none
+    uint64_t ja[2] = {0, 0}, jc[2] = {0, 0};
+    if (tid < 3) {
+        ja[0] = S.ja1[2 * tid];
+        jc[0] = S.jc1[2 * tid];
+        if (tid < 2) {
+            ja[1] = S.ja1[2 * tid + 1];
+            jc[1] = S.jc1[2 * tid + 1];
+        }
+    }
     // pair terms of the mover at its current position (lanes < NUP)
     R ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};
     bool have_old = false;
@@ -412,22 +438,23 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             g1 = wave_sum(g1);
         }
         const double go[3] = {double(g0), double(g1), double(wave_sum(g2))};
-        // gaussian(3) then the acceptance draw: five ran1 draws per move
-        double u[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            s = (s * kM1 + kA1) & kMask48;
-            u[k] = double(s | 1ull) * 0x1p-48;
-        }
-        double g3[3];
+        // gaussian(3) then the acceptance draw: five ran1 draws per move, formed by jump-ahead
+        // from the walker state in separate lanes (lane l < 2: the Box-Muller pair of draws
+        // 2l+1, 2l+2; lane 2: the fifth draw, the walker's next state) and broadcast
+        double g3[3], u4;
         {
-            R s1, c1, s3, c3;
-            sincos_2pi<R>(u[0], s1, c1);
-            sincos_2pi<R>(u[2], s3, c3);
-            const R r12 = sqrt(R(-2) * log(R(u[1]))), r34 = sqrt(R(-2) * log(R(u[3])));
-            g3[0] = double(s1 * r12);
-            g3[1] = double(c1 * r12);
-            g3[2] = double(c3 * r34);
+            const uint64_t sa = (ja[0] * s + jc[0]) & kMask48, sb = (ja[1] * s + jc[1]) & kMask48;
+            const double ua = double(sa | 1ull) * 0x1p-48, ub = double(sb | 1ull) * 0x1p-48;
+            R sn, cs;
+            if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * ua, &sn, &cs);
+            else sincos_turns(float(ua - rint(ua)), sn, cs);
+            const R rad = sqrt(R(-2) * log(R(ub)));
+            const R gx = sn * rad, gy = cs * rad;
+            g3[0] = double(lane_value(gx, 0));
+            g3[1] = double(lane_value(gy, 0));
+            g3[2] = double(lane_value(gy, 1));
+            s = lane_value(sa, 2);
+            u4 = double(s | 1ull) * 0x1p-48;
         }
         double dro[3], dxm[3], xn[3];
 #pragma unroll
@@ -526,11 +553,11 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             }
             if constexpr (sizeof(R) == 8) {
                 const double logp = 2.0 * (log(fabs(rho)) + dlogj) + lq * half_isig2;
-                accept = u[4] < exp(fmin(0.0, logp));
+                accept = u4 < exp(fmin(0.0, logp));
             } else {
                 const float logp = 2.0f * (logf(fabsf(float(rho))) + float(dlogj)) +
                                    float(lq * half_isig2);
-                accept = u[4] < double(expf(fminf(0.0f, logp)));
+                accept = u4 < double(expf(fminf(0.0f, logp)));
             }
         }
         if (accept) {
