@@ -168,16 +168,17 @@ template <int NUP, typename R>
 struct MvLayout {
     static constexpr int NP = 2 * NUP;
     static constexpr int RD = sizeof(R) == 8 ? 1 : 2;  // R values per double
-    // row stride (complex values) of the phase planes P[q][p] and of the transposition
-    // area: odd, so that lanes reading one partner at different q, or one row of the
-    // transposition each, hit distinct banks
-    static constexpr int ST = NUP | 1;
     // the column broadcast is split in two halves (partners [0, HB) and [HB, NUP)), each
     // starting on a 16-byte boundary
     static constexpr int HB = (NUP + 1) / 2, HBP = (HB + 3) / 4 * 4;
+    // row stride (complex values) of the phase planes P[q][p] and of the transposition
+    // area: odd, so that lanes reading one partner at different q, or one row of the
+    // transposition each, hit distinct banks; in FP32 also at least 2 HB, so that the two
+    // half-waves of the projection read whole halves (the columns past NUP are zero)
+    static constexpr int ST = sizeof(R) == 8 ? (NUP | 1) : ((2 * HB) | 1);
     static constexpr int NB = (NUP + 3) / 4 * 4;
     static __host__ __device__ constexpr int dbl(int nr) { return ((nr + RD - 1) / RD + 1) / 2 * 2; }
-    int xs, pu, pd, qk, qm, wb, fb, ub, total;
+    int xs, pu, pd, qk, qm, wb, fb, ub, jmp, total;
     __host__ __device__ explicit MvLayout(int nq) {
         xs = 0;                                  // positions, [NP][3] FP64
         const int plane = dbl(2 * nq * ST);      // phases of one spin, [nq][ST] complex
@@ -190,7 +191,8 @@ struct MvLayout {
         wb = qm + 2 * nq;                        // inverse column, two halves (R)
         fb = wb + dbl(2 * HBP);                  // new pair values (R)
         ub = fb + dbl(NB);                       // w_q exp(i q.x') (R)
-        total = ub + dbl(2 * nq + 4);
+        jmp = ub + dbl(2 * nq + 4);              // jump-ahead a, c of draws 1..5 (uint64)
+        total = jmp + 10;
     }
 };
 
@@ -223,8 +225,14 @@ __device__ __forceinline__ void move_phases(const DevSys &S, const double *xs, c
                                             typename Vec2<R>::T *pu, typename Vec2<R>::T *pd,
                                             int tid) {
     using R2 = typename Vec2<R>::T;
-    constexpr int NP = 2 * NUP, ST = NUP | 1;
+    constexpr int NP = 2 * NUP, ST = MvLayout<NUP, R>::ST, PADC = ST - NUP;
     const int nq = S.nq;
+    for (int t = tid; t < 2 * nq * PADC; t += 64) {
+        const int row = t / PADC, col = NUP + (t - row * PADC);
+        R2 z;
+        z.x = z.y = R(0);
+        (row < nq ? pu : pd)[(row < nq ? row : row - nq) * ST + col] = z;
+    }
     for (int p = tid; p < NP; p += 64) {
         R2 *pl = p < NUP ? pu + p : pd + (p - NUP);
         R c[3][5], sn[3][5];
@@ -338,21 +346,21 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
     }
     wave_sync();
 
+    for (int t = tid; t < 2 * HBP; t += 64) wb[t] = R(0);  // padding of the second half stays 0
     const double sig = S.sigma, sig2 = S.sig2, half_isig2 = 0.5 / S.sig2;
     const R alpha0 = R(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
     // jump-ahead coefficients of this lane's draws (state after t + 1 draws = ja1[t] s +
-    // jc1[t]): lane l < 2 draws 2l, 2l + 1; lane 2 draw 4; other lanes none
-    uint64_t ja[2] = {0, 0}, jc[2] = {0, 0};
-    if (tid < 3) {
-        ja[0] = S.ja1[2 * tid];
-        jc[0] = S.jc1[2 * tid];
-        if (tid < 2) {
-            ja[1] = S.ja1[2 * tid + 1];
-            jc[1] = S.jc1[2 * tid + 1];
-        }
+    // jc1[t]): lane l < 2 draws 2l, 2l + 1; lane 2 draw 4; other lanes none (zero, the draw
+    // then stays finite and unused).  Staged in LDS, read where used.
+    uint64_t *jmp = reinterpret_cast<uint64_t *>(sm + X.jmp);
+    if (tid < 5) {
+        jmp[tid] = S.ja1[tid];
+        jmp[5 + tid] = S.jc1[tid];
     }
+    wave_sync();
+    const int jl0 = tid < 3 ? 2 * tid : -1, jl1 = tid < 2 ? 2 * tid + 1 : -1;
     // pair terms of the mover at its current position (lanes < NUP)
     R ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};
     bool have_old = false;
@@ -402,12 +410,18 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             const int o = ol + 32 * c;
             R acc = R(0);
             if (o < no) {
-                const R *pl = reinterpret_cast<const R *>(Pp + (o >> 1) * ST) + (o & 1);
+                const R *pl = reinterpret_cast<const R *>(Pp + (o >> 1) * ST + h * HB) + (o & 1);
                 const R *wh = wb + h * HBP;
-                const int b0 = h * HB;
+                if constexpr (ST >= 2 * HB) {
+                    // padded rows (FP32): both half-waves run all HB terms
 #pragma unroll
-                for (int bb = 0; bb < HB; ++bb)
-                    if (b0 + bb < NUP) acc += wh[bb] * pl[2 * (b0 + bb)];
+                    for (int bb = 0; bb < HB; ++bb) acc += wh[bb] * pl[2 * bb];
+                } else {
+                    // unpadded rows (FP64, where the padding would cost occupancy)
+#pragma unroll
+                    for (int bb = 0; bb < HB; ++bb)
+                        if (h * HB + bb < NUP) acc += wh[bb] * pl[2 * bb];
+                }
             }
             Xo[c] = half_sum(acc);
         }
@@ -443,7 +457,9 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         // 2l+1, 2l+2; lane 2: the fifth draw, the walker's next state) and broadcast
         double g3[3], u4;
         {
-            const uint64_t sa = (ja[0] * s + jc[0]) & kMask48, sb = (ja[1] * s + jc[1]) & kMask48;
+            const uint64_t ja0 = jl0 >= 0 ? jmp[jl0] : 0, jc0 = jl0 >= 0 ? jmp[5 + jl0] : 0;
+            const uint64_t ja1 = jl1 >= 0 ? jmp[jl1] : 0, jc1 = jl1 >= 0 ? jmp[5 + jl1] : 0;
+            const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
             const double ua = double(sa | 1ull) * 0x1p-48, ub = double(sb | 1ull) * 0x1p-48;
             R sn, cs;
             if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * ua, &sn, &cs);
@@ -543,20 +559,31 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         const double dlogj = double(dU);
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
-            const double irho = 1.0 / rho;
-            double lq = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                double drn = (gn[d] * irho + jn[d]) * sig2;
-                if (S.cutdrift) drn = fmin(S.dcut, fmax(-S.dcut, drn));
-                lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
-            }
+            // Metropolis exponent of the drift-diffusion Green's functions: FP64 moves in FP64,
+            // mixed moves in FP32 (its inputs carry FP32 errors already)
             if constexpr (sizeof(R) == 8) {
+                const double irho = 1.0 / rho;
+                double lq = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double drn = (gn[d] * irho + jn[d]) * sig2;
+                    if (S.cutdrift) drn = fmin(S.dcut, fmax(-S.dcut, drn));
+                    lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
+                }
                 const double logp = 2.0 * (log(fabs(rho)) + dlogj) + lq * half_isig2;
                 accept = u4 < exp(fmin(0.0, logp));
             } else {
+                const float irho = 1.0f / float(rho), fs2 = float(sig2), fcut = float(S.dcut);
+                float lq = 0.0f;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    float drn = (float(gn[d]) * irho + float(jn[d])) * fs2;
+                    if (S.cutdrift) drn = fminf(fcut, fmaxf(-fcut, drn));
+                    const float dof = float(dro[d]);
+                    lq += (dof + drn) * (dof - drn - 2.0f * float(dxm[d]));
+                }
                 const float logp = 2.0f * (logf(fabsf(float(rho))) + float(dlogj)) +
-                                   float(lq * half_isig2);
+                                   lq * float(half_isig2);
                 accept = u4 < double(expf(fminf(0.0f, logp)));
             }
         }
