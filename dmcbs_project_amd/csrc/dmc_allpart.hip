@@ -28,8 +28,9 @@ __global__ __launch_bounds__(64, 2) void k_eval(DevSys S, const double *__restri
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = x[size_t(w) * M + t];
     wave_sync();
     const Tile<NUP> T(tid);
-    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid);
-    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
+    PairTile<NUP> rad;
+    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid, &rad);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid, &rad);
     if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_bytes(S), tid);
     grad_lap<NUP>(S, sm, L, T, D, J, tid);
     for (int t = tid; t < M; t += 64) dpsi[size_t(w) * M + t] = sm[L.dp + t];

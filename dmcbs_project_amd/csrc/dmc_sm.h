@@ -669,8 +669,9 @@ __device__ __forceinline__ void sm_end_walker(const DevSys &S, double *sm, int w
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = B.x[size_t(w) * M + t];
     wave_sync();
     const Tile<NUP> Tl(tid);
-    const JSums J = jastrow_terms<NUP>(S, sm, L, Tl, tid);
-    const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid);
+    PairTile<NUP> rad;
+    const JSums J = jastrow_terms<NUP>(S, sm, L, Tl, tid, &rad);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid, &rad);
     store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_bytes(S), tid);
     grad_lap<NUP>(S, sm, L, Tl, D, J, tid);
     finish_walker<NUP>(S, sm, L, A, B, T, w, tid, B.irn[w], opcomp);

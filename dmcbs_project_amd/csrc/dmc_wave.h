@@ -288,6 +288,56 @@ struct Tile {
     }
 };
 
+// Argmax of (|pivot| candidate, row) over the lanes that share lane & 7 (the 8 lanes of a
+// tile column group), ties to the lower row; every lane of the group ends with the result.
+// The exchanges are VALU permutes: DPP row_ror:8 (lane ^ 8) and the permlane16/32 swaps
+// (lane ^ 16, lane ^ 32).  A swap of a register with itself returns the same ordered pair of
+// values in both lanes of a partner pair, and the xor-8 pair is put in lane order, so that the
+// choice is made identically in both lanes (NaN candidates included).
+__device__ __forceinline__ void pick_pivot(double b1, int r1, double b2, int r2, double &b,
+                                           int &r) {
+    const bool t = b2 > b1 || (b2 == b1 && r2 < r1);
+    b = t ? b2 : b1;
+    r = t ? r2 : r1;
+}
+__device__ __forceinline__ void pivot_argmax(double &best, int &brow, int tid) {
+    {
+        const long long bb = __double_as_longlong(best);
+        const int lo = __builtin_amdgcn_update_dpp(0, int(bb & 0xffffffffll), 0x128, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, int(bb >> 32), 0x128, 0xF, 0xF, false);
+        const int orow = __builtin_amdgcn_update_dpp(0, brow, 0x128, 0xF, 0xF, false);
+        const double ob = dbl_of(lo, hi);
+        if (tid & 8) pick_pivot(ob, orow, best, brow, best, brow);
+        else pick_pivot(best, brow, ob, orow, best, brow);
+    }
+    {
+        const long long bb = __double_as_longlong(best);
+        const unsigned lo = unsigned(bb & 0xffffffffll), hi = unsigned(bb >> 32);
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        const auto rr = __builtin_amdgcn_permlane16_swap(unsigned(brow), unsigned(brow), false,
+                                                         false);
+        pick_pivot(dbl_of(int(l[0]), int(h[0])), int(rr[0]), dbl_of(int(l[1]), int(h[1])),
+                   int(rr[1]), best, brow);
+    }
+    {
+        const long long bb = __double_as_longlong(best);
+        const unsigned lo = unsigned(bb & 0xffffffffll), hi = unsigned(bb >> 32);
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        const auto rr = __builtin_amdgcn_permlane32_swap(unsigned(brow), unsigned(brow), false,
+                                                         false);
+        pick_pivot(dbl_of(int(l[0]), int(h[0])), int(rr[0]), dbl_of(int(l[1]), int(h[1])),
+                   int(rr[1]), best, brow);
+    }
+}
+
+// the radial part alpha_1 + beta(r) of a lane's tile of the pairing matrix
+template <int NUP>
+struct PairTile {
+    double v[Layout<NUP>::TR][Layout<NUP>::TR];
+};
+
 struct DetInfo {
     double logdet;
     int sgn;
@@ -298,8 +348,9 @@ struct DetInfo {
 // inverse, log|det| and sign.  Leaves the phases (work area) and the inverse (sinv,
 // Sinv(j, i) at [j * SP + i]) in LDS.
 template <int NUP>
-__device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> &L,
-                                 const Tile<NUP> &T, int tid) {
+__device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
+                                                 const Layout<NUP> &L, const Tile<NUP> &T,
+                                                 int tid, const PairTile<NUP> *rad = nullptr) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
     const int nq = S.nq;
     const int r = T.r, c = T.c;
@@ -317,12 +368,17 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
     phases_all<NUP>(S, xs, qm, ph, tid);
     wave_sync();
 
-    // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij)
+    // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij);
+    // the radial part alpha_1 + beta comes from the Jastrow pass when it ran first (rad)
     double Sm[TR][TR];
 #pragma unroll
     for (int a = 0; a < TR; ++a)
 #pragma unroll
         for (int b = 0; b < TR; ++b) {
+            if (rad) {
+                Sm[a][b] = rad->v[a][b];
+                continue;
+            }
             const double *xi = xs + 3 * ri[a], *xj = xs + 3 * (NUP + cj[b]);
             double r2 = 0.0;
 #pragma unroll
@@ -385,15 +441,7 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
                 }
             }
         }
-#pragma unroll
-        for (int m = 8; m <= 32; m <<= 1) {
-            const double ob = __shfl_xor(best, m, 64);
-            const int orow = __shfl_xor(brow, m, 64);
-            if (ob > best || (ob == best && orow < brow)) {
-                best = ob;
-                brow = orow;
-            }
-        }
+        pivot_argmax(best, brow, tid);
         const int p = __builtin_amdgcn_readlane(brow, ck);
         const int ap = p >> 3;
         if (r == (p & 7)) {
@@ -490,15 +538,17 @@ __device__ DetInfo build_inverse(const DevSys &S, double *sm, const Layout<NUP> 
 // spins: per particle, lane p < NUP holds the Jastrow gradient of up particle p (j[0]) and
 // down particle p (j[1]); the sums U, V and d2U are wave-uniform.  Runs before the pairing
 // matrix is built (the lean layout's work area is free then), so that its partial sums are
-// never live together with the inverse.
+// never live together with the inverse; with rad it also gives the radial part of the
+// lane's tile of the pairing matrix from the same pair distances.
 struct JSums {
     double j[2][3];
     double U, V, D2U;
 };
 
 template <int NUP>
-__device__ JSums jastrow_terms(const DevSys &S, double *sm, const Layout<NUP> &L,
-                               const Tile<NUP> &T, int tid) {
+__device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
+                                               const Layout<NUP> &L, const Tile<NUP> &T,
+                                               int tid, PairTile<NUP> *rad = nullptr) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP;
     constexpr int PS = 3 * 8 * SP;
     const double *xs = sm + L.xs;
@@ -525,7 +575,8 @@ __device__ JSums jastrow_terms(const DevSys &S, double *sm, const Layout<NUP> &L
                 dx[d] -= S.el * rint(dx[d] * S.eli);
                 r2 += dx[d] * dx[d];
             }
-            const double rj = fmax(1.0e-14, sqrt(r2));
+            const double rr = sqrt(r2), rj = fmax(1.0e-14, rr);
+            if (rad) rad->v[a][b] = S.alpha0 + beta_val(S, rr);
             if (rj < S.range) {
                 double w0, w1, w2, w3;
                 const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
@@ -814,8 +865,9 @@ template <int NUP>
 __device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
                                            int tid) {
     const Tile<NUP> T(tid);
-    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid);
-    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid);
+    PairTile<NUP> rad;
+    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid, &rad);
+    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid, &rad);
     grad_lap<NUP>(S, sm, L, T, D, J, tid);
 }
 
