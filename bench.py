@@ -133,6 +133,36 @@ def elocal_error(nconf=4096, krefresh=N):
                 matched=int(len(common)), walkers=int(len(w0["irn"])))
 
 
+def energy_agreement(nequil=200, nblocks=20, blocklen=20):
+    """DMC energy of the mixed and the FP64 propagators on the headline workload: both start
+    from the same walkers (seed 7) and random streams, equilibrate nequil steps, then sample
+    the energy estimator (E_L / (N E_FG), weighted, estimator.f90:72-82) on every step of
+    nblocks blocks of blocklen steps.  Block means give each mean and its standard error;
+    the two runs are independent Markov chains after a few steps, so their difference is
+    compared with the combined error."""
+    out = {}
+    for mode in ("mixed", "fp64"):
+        e = make_engine(mode)
+        for _ in range(nequil):
+            e.step()
+        e.estimators(reset=True)
+        blocks = []
+        for _ in range(nblocks):
+            for _ in range(blocklen):
+                e.step(opcomp=True)
+            est = e.estimators(reset=True)
+            blocks.append(est[0, 0] / est[0, 2])
+        b = np.array(blocks)
+        out[mode] = (float(b.mean()), float(b.std(ddof=1) / np.sqrt(nblocks)))
+        e.close()
+    (m32, s32), (m64, s64) = out["mixed"], out["fp64"]
+    sig = float(np.hypot(s32, s64))
+    return dict(e_mixed=m32, err_mixed=s32, e_fp64=m64, err_fp64=s64, diff=m32 - m64,
+                diff_over_sigma=(m32 - m64) / sig if sig > 0 else None,
+                steps=f"{nequil} equilibration + {nblocks} x {blocklen} sampled",
+                unit="E_L / (N E_FG)")
+
+
 BATCH_SYSTEMS = [(14, 3, 8192), (38, 4, 16384), (66, 5, 24576)]
 
 
@@ -323,6 +353,8 @@ def main():
         extra["mixed_over_fp64"] = main_run["value"] / fp["value"]
         extra["elocal_rel_err"] = elocal_error()
         extra["elocal_rel_err_k"] = {str(k): elocal_error(krefresh=k) for k in (16, 33)}
+        extra["dmc_energy"] = energy_agreement()
+        print(f"bench: energy: {extra['dmc_energy']}", file=sys.stderr, flush=True)
         br = batch_run(min(a.steps, 30), 3)
         print(f"bench: batch: {br}", file=sys.stderr, flush=True)
         extra["batch"] = br
