@@ -11,6 +11,7 @@
 namespace dmc {
 
 constexpr int kScanThreads = 1024;
+constexpr int kEstBlocks = 64;  // blocks of the estimator sums
 
 // Device population words of DevStep::nt: [0] walkers a kernel processes, [1] total after a
 // scan, [2] resident population (device-driven steps), [3] overflow flag, [4] population
@@ -35,30 +36,63 @@ __global__ void k_branch_count(DevState A, DevStep T, double factor, const int *
     if (iwt >= 0 && iwt < 500) atomicAdd(reinterpret_cast<unsigned long long *>(&T.hist[iwt]), 1ull);
 }
 
-// exclusive scan of count[0..nt[0]) into offset, total into nt[1]; one block
+// exclusive scan of count[0..nt[0]) into offset, total into nt[1]; one block.  Thread t
+// owns a contiguous chunk, read and written in 16-byte vectors where it is aligned; the
+// chunk sums are scanned by wave shuffles and the 16 wave totals in LDS.
 __global__ __launch_bounds__(kScanThreads) void k_scan(DevStep T, const int *gate) {
     if (gated_off(gate)) return;
-    __shared__ int part[kScanThreads];
+    __shared__ int wtot[kScanThreads / 64];
     const int n = T.nt[0];
-    const int tid = threadIdx.x;
-    const int chunk = (n + kScanThreads - 1) / kScanThreads;
-    const int lo = tid * chunk, hi = min(n, lo + chunk);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int chunk = ((n + kScanThreads - 1) / kScanThreads + 3) & ~3;  // multiple of 4
+    const int lo = min(n, tid * chunk), hi = min(n, lo + chunk);
     int s = 0;
-    for (int i = lo; i < hi; ++i) s += T.count[i];
-    part[tid] = s;
+    {
+        int i = lo;
+        for (; i + 4 <= hi; i += 4) {
+            const int4 c = *reinterpret_cast<const int4 *>(T.count + i);
+            s += c.x + c.y + c.z + c.w;
+        }
+        for (; i < hi; ++i) s += T.count[i];
+    }
+    // inclusive scan of the chunk sums over the wave, then over the waves
+    int incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wtot[wv] = incl;
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {  // Hillis-Steele inclusive scan
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    if (wv == 0) {
+        int t = lane < kScanThreads / 64 ? wtot[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < kScanThreads / 64; off <<= 1) {
+            const int y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
+        }
+        if (lane < kScanThreads / 64) wtot[lane] = t;
     }
-    int run = tid ? part[tid - 1] : 0;
-    for (int i = lo; i < hi; ++i) {
-        T.offset[i] = run;
-        run += T.count[i];
+    __syncthreads();
+    int run = incl - s + (wv ? wtot[wv - 1] : 0);
+    {
+        int i = lo;
+        for (; i + 4 <= hi; i += 4) {
+            const int4 c = *reinterpret_cast<const int4 *>(T.count + i);
+            int4 o;
+            o.x = run;
+            o.y = o.x + c.x;
+            o.z = o.y + c.y;
+            o.w = o.z + c.z;
+            run = o.w + c.w;
+            *reinterpret_cast<int4 *>(T.offset + i) = o;
+        }
+        for (; i < hi; ++i) {
+            T.offset[i] = run;
+            run += T.count[i];
+        }
     }
-    if (tid == kScanThreads - 1) T.nt[1] = part[kScanThreads - 1];
+    if (tid == kScanThreads - 1) T.nt[1] = wtot[kScanThreads / 64 - 1];
 }
 
 // destination map and the RNG state of every copy
@@ -107,21 +141,43 @@ __global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState 
     }
 }
 
-// est_acc[k] += sum_w est[w][k], k < 42, in a fixed order (one block per component);
-// nw < 0: the count in nt[0]
+// Estimator sums over walkers 0..nw-1 (nw < 0: nt[0]) added to acc[0..41]: block b sums a
+// contiguous range of walkers, reading rows of 42 values with consecutive threads, and the
+// last block to finish adds the block partials in block order.  Both orders are fixed, so
+// the sums do not change from run to run.  nt[5] counts finished blocks (reset by the last).
 __global__ __launch_bounds__(256) void k_est_reduce(DevStep T, int nw, double *acc) {
-    __shared__ double part[256];
-    const int k = blockIdx.x, tid = threadIdx.x;
+    __shared__ double part[6][42];
+    __shared__ int last;
     if (nw < 0) nw = T.nt[0];
-    double s = 0.0;
-    for (int w = tid; w < nw; w += 256) s += T.est[size_t(w) * 42 + k];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) part[tid] += part[tid + off];
-        __syncthreads();
+    const int tid = threadIdx.x, b = blockIdx.x;
+    const int chunk = (nw + kEstBlocks - 1) / kEstBlocks;
+    const int w0 = min(nw, b * chunk), w1 = min(nw, w0 + chunk);
+    const int g = tid / 42, k = tid - 42 * g;
+    if (g < 6) {
+        double s = 0.0;
+        for (int w = w0 + g; w < w1; w += 6) s += T.est[size_t(w) * 42 + k];
+        part[g][k] = s;
     }
-    if (tid == 0) acc[k] += part[0];
+    __syncthreads();
+    if (tid < 42) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v += part[j][tid];
+        T.estp[b * 42 + tid] = v;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(T.nt + 5, 1) == kEstBlocks - 1;
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        if (tid < 42) {
+            double v = 0.0;
+            for (int j = 0; j < kEstBlocks; ++j) v += T.estp[j * 42 + tid];
+            acc[tid] += v;
+        }
+        if (tid == 0) T.nt[5] = 0;
+    }
 }
 
 // calder (optimizer.f90:90-112) over walkers 0..nw-1: acc[k] += sum_w term_k(w) in a fixed
@@ -272,7 +328,7 @@ int launch_pop_begin(const DevStep &T, hipStream_t st) {
 }
 
 int launch_est_reduce(const DevStep &T, int grid, double *acc, hipStream_t st) {
-    hipLaunchKernelGGL(k_est_reduce, dim3(42), dim3(256), 0, st, T, grid, acc);
+    hipLaunchKernelGGL(k_est_reduce, dim3(kEstBlocks), dim3(256), 0, st, T, grid, acc);
     return hipGetLastError();
 }
 

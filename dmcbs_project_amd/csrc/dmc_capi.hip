@@ -293,7 +293,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         c->alloc_state(c->A) || c->alloc_state(c->B) || c->alloc(&c->T.count, c->capacity) ||
         c->alloc(&c->T.offset, c->capacity) || c->alloc(&c->T.map, c->capacity) ||
         c->alloc(&c->T.copy_irn, c->capacity) || c->alloc(&c->T.est, size_t(c->capacity) * 42) ||
-        c->alloc(&c->T.nt, 8) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
+        c->alloc(&c->T.nt, 8) || c->alloc(&c->T.estp, 64 * 42) || c->alloc(&c->T.hist, 500) || c->alloc(&c->T.nacc, c->capacity) ||
         c->alloc(&c->T.moves, 2) ||
         c->alloc(&c->est_acc, 42) || c->alloc(&c->opt_acc, dmc_ctx::kOptSums)) {
         std::string e = g_err;
@@ -328,6 +328,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         hipMemcpy(ja2, a2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(jc2, c2.data(), 8 * kMaxDraw, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->est_acc, 0, 42 * sizeof(double)) != hipSuccess ||
+        hipMemset(c->T.nt, 0, 8 * sizeof(int)) != hipSuccess ||
         hipMemset(c->opt_acc, 0, dmc_ctx::kOptSums * sizeof(double)) != hipSuccess ||
         hipMemset(c->T.hist, 0, 500 * sizeof(int64_t)) != hipSuccess ||
         hipMemset(c->T.moves, 0, 2 * sizeof(int64_t)) != hipSuccess)

@@ -45,6 +45,7 @@ struct DevStep {
     int *map;             // destination -> source walker
     uint64_t *copy_irn;   // destination RNG state
     double *est;          // per-walker estimator contributions [w][14][3]
+    double *estp;         // block partials of the estimator sums [64][42]
     int *nt;              // device population counters: nt[0] current, nt[1] new
     int64_t *hist;        // branch multiplicity histogram [500]
     int *nacc;            // accepted single-particle moves per walker
