@@ -51,9 +51,26 @@ __device__ __forceinline__ void sincos_turns(float t, float &s, float &c) {
     cp = fmaf(x2, cp, 4.16666667e-2f);
     cp = fmaf(x2, cp, -0.5f);
     const float cs = fmaf(x2, cp, 1.0f);
-    const int iq = int(q) & 3;
-    s = iq == 0 ? sn : iq == 1 ? cs : iq == 2 ? -sn : -cs;
-    c = iq == 0 ? cs : iq == 1 ? -sn : iq == 2 ? -cs : sn;
+    // quarter turn iq (mod 4): odd iq swaps sin and cos; sin changes sign for iq = 2, 3 and
+    // cos for iq = 1, 2 (selects and sign flips, no branches)
+    const unsigned iq = unsigned(int(q));
+    const bool sw = iq & 1u;
+    const float s0 = sw ? cs : sn, c0 = sw ? sn : cs;
+    s = __uint_as_float(__float_as_uint(s0) ^ ((iq & 2u) << 30));
+    c = __uint_as_float(__float_as_uint(c0) ^ (((iq ^ (iq >> 1)) & 1u) << 31));
+}
+
+// natural log and exp: the library functions in FP64, the hardware log2/exp2 (about 1 ulp)
+// scaled by ln 2 / log2 e in FP32
+template <typename R>
+__device__ __forceinline__ R log_r(R x) {
+    if constexpr (sizeof(R) == 8) return log(x);
+    else return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+}
+template <typename R>
+__device__ __forceinline__ R exp_r(R x) {
+    if constexpr (sizeof(R) == 8) return exp(x);
+    else return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
 }
 
 // sin and cos of 2 pi t for a turn count t given in FP64: FP64 reduction to the nearest
@@ -80,14 +97,21 @@ __device__ __forceinline__ uint64_t lane_value(uint64_t v, int l) {
     return (uint64_t(hi) << 32) | lo;
 }
 
-// distance of the pair (a, b), the minimum-image difference formed in FP64 and the rest in R
+// distance of the pair (a, b): the difference of the FP64 positions, then the minimum image
+// in FP64 (R = double) or, rounded to FP32, in FP32 (the wrapped difference then carries an
+// absolute error of about ulp(L))
 template <typename R>
 __device__ __forceinline__ R pair_dist(const DevSys &S, const double *xa, const double *xb,
                                        R (&dx)[3]) {
     R r2 = R(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        dx[d] = R(wrapr(xa[d] - xb[d], S));
+        if constexpr (sizeof(R) == 8) {
+            dx[d] = wrapr(xa[d] - xb[d], S);
+        } else {
+            const float t = float(xa[d] - xb[d]);
+            dx[d] = fmaf(-float(S.el), rintf(t * float(S.eli)), t);
+        }
         r2 += dx[d] * dx[d];
     }
     return sqrt_r<R>(r2);
@@ -227,11 +251,13 @@ __device__ __forceinline__ void move_phases(const DevSys &S, const double *xs, c
     using R2 = typename Vec2<R>::T;
     constexpr int NP = 2 * NUP, ST = MvLayout<NUP, R>::ST, PADC = ST - NUP;
     const int nq = S.nq;
-    for (int t = tid; t < 2 * nq * PADC; t += 64) {
-        const int row = t / PADC, col = NUP + (t - row * PADC);
-        R2 z;
-        z.x = z.y = R(0);
-        (row < nq ? pu : pd)[(row < nq ? row : row - nq) * ST + col] = z;
+    if constexpr (PADC > 0) {
+        for (int t = tid; t < 2 * nq * PADC; t += 64) {
+            const int row = t / PADC, col = NUP + (t - row * PADC);
+            R2 z;
+            z.x = z.y = R(0);
+            (row < nq ? pu : pd)[(row < nq ? row : row - nq) * ST + col] = z;
+        }
     }
     for (int p = tid; p < NP; p += 64) {
         R2 *pl = p < NUP ? pu + p : pd + (p - NUP);
@@ -464,7 +490,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             R sn, cs;
             if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * ua, &sn, &cs);
             else sincos_turns(float(ua - rint(ua)), sn, cs);
-            const R rad = sqrt(R(-2) * log(R(ub)));
+            const R rad = sqrt_r<R>(R(-2) * log_r<R>(R(ub)));
             const R gx = sn * rad, gy = cs * rad;
             g3[0] = double(lane_value(gx, 0));
             g3[1] = double(lane_value(gy, 0));
@@ -505,7 +531,8 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
 #pragma unroll 4
             for (int q = 0; q < nq; ++q) {
                 const R2 v = pp[q * ST];
-                f += ub[2 * q] * v.x + ub[2 * q + 1] * v.y;
+                f = fma(ub[2 * q], v.x, f);
+                f = fma(ub[2 * q + 1], v.y, f);
             }
             fnew = f;
             rho_part = double(wgt) * double(f);
@@ -573,7 +600,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                 const double logp = 2.0 * (log(fabs(rho)) + dlogj) + lq * half_isig2;
                 accept = u4 < exp(fmin(0.0, logp));
             } else {
-                const float irho = 1.0f / float(rho), fs2 = float(sig2), fcut = float(S.dcut);
+                const float irho = rcp_r<float>(float(rho)), fs2 = float(sig2), fcut = float(S.dcut);
                 float lq = 0.0f;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -582,14 +609,14 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                     const float dof = float(dro[d]);
                     lq += (dof + drn) * (dof - drn - 2.0f * float(dxm[d]));
                 }
-                const float logp = 2.0f * (logf(fabsf(float(rho))) + float(dlogj)) +
+                const float logp = 2.0f * (log_r<float>(fabsf(float(rho))) + float(dlogj)) +
                                    lq * float(half_isig2);
-                accept = u4 < double(expf(fminf(0.0f, logp)));
+                accept = u4 < double(exp_r<float>(fminf(0.0f, logp)));
             }
         }
         if (accept) {
             ++nacc;
-            const R irr = R(1.0 / rho);
+            const R irr = rcp_r<R>(R(rho));
             if (tid < NUP) fb[tid] = fnew;
             wave_sync();
             if (tid < NUP) {
