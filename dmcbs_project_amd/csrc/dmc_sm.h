@@ -142,11 +142,8 @@ __device__ __forceinline__ typename Vec4<R>::T pair_tab(const DevSys &S, R r) {
     int idx = int(s);
     idx = idx > ntab - 2 ? ntab - 2 : idx;
     idx = idx < 1 ? 1 : idx;
-    const R t = s - R(idx);
-    const R w0 = -t * (t - R(1)) * (t - R(2)) * R(1.0 / 6.0);
-    const R w1 = (t + R(1)) * (t - R(1)) * (t - R(2)) * R(0.5);
-    const R w2 = -(t + R(1)) * t * (t - R(2)) * R(0.5);
-    const R w3 = (t + R(1)) * t * (t - R(1)) * R(1.0 / 6.0);
+    R w0, w1, w2, w3;
+    lagrange_weights<R>(s - R(idx), w0, w1, w2, w3);
     const R4 *tab;
     if constexpr (sizeof(R) == 8) tab = S.mt64;
     else tab = S.mt32;
@@ -374,6 +371,8 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
 
     for (int t = tid; t < 2 * HBP; t += 64) wb[t] = R(0);  // padding of the second half stays 0
     const double sig = S.sigma, sig2 = S.sig2, half_isig2 = 0.5 / S.sig2;
+    // drift cut (an infinite cut when the drift is not cut)
+    const double dcut = S.cutdrift ? S.dcut : __builtin_huge_val();
     const R alpha0 = R(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
@@ -477,11 +476,12 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             g0 = wave_sum(g0);
             g1 = wave_sum(g1);
         }
-        const double go[3] = {double(g0), double(g1), double(wave_sum(g2))};
+        const R go[3] = {g0, g1, wave_sum(g2)};
         // gaussian(3) then the acceptance draw: five ran1 draws per move, formed by jump-ahead
         // from the walker state in separate lanes (lane l < 2: the Box-Muller pair of draws
         // 2l+1, 2l+2; lane 2: the fifth draw, the walker's next state) and broadcast
-        double g3[3], u4;
+        R g3[3];
+        double u4;
         {
             const uint64_t ja0 = jl0 >= 0 ? jmp[jl0] : 0, jc0 = jl0 >= 0 ? jmp[5 + jl0] : 0;
             const uint64_t ja1 = jl1 >= 0 ? jmp[jl1] : 0, jc1 = jl1 >= 0 ? jmp[5 + jl1] : 0;
@@ -492,19 +492,24 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             else sincos_turns(float(ua - rint(ua)), sn, cs);
             const R rad = sqrt_r<R>(R(-2) * log_r<R>(R(ub)));
             const R gx = sn * rad, gy = cs * rad;
-            g3[0] = double(lane_value(gx, 0));
-            g3[1] = double(lane_value(gy, 0));
-            g3[2] = double(lane_value(gy, 1));
+            g3[0] = lane_value(gx, 0);
+            g3[1] = lane_value(gy, 0);
+            g3[2] = lane_value(gy, 1);
             s = lane_value(sa, 2);
             u4 = double(s | 1ull) * 0x1p-48;
         }
-        double dro[3], dxm[3], xn[3];
+        // drift and displacement in R (FP32 for the mixed moves, whose gradient and draws are
+        // FP32), added to the FP64 position
+        R dro[3], dxm[3];
+        double xn[3];
+        {
+            const R rs2 = R(sig2), rsig = R(sig), rcut = R(dcut);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            dro[d] = go[d] * sig2;
-            if (S.cutdrift) dro[d] = fmin(S.dcut, fmax(-S.dcut, dro[d]));
-            dxm[d] = dro[d] + sig * g3[d];
-            xn[d] = wrapr(xa[d] + dxm[d], S);
+            for (int d = 0; d < 3; ++d) {
+                dro[d] = fmin(rcut, fmax(-rcut, go[d] * rs2));
+                dxm[d] = fma(rsig, g3[d], dro[d]);
+                xn[d] = wrapr(xa[d] + double(dxm[d]), S);
+            }
         }
         // phases of the proposed position, one q per lane pair: the argument reduced to
         // a fraction of a turn in FP64
@@ -567,56 +572,50 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             have_old = true;
         }
         const double rho = wave_sum(rho_part);  // FP64 sum: the node test uses its sign
+        // grad ln psi at the proposed position, sum_b (g_b / rho + j_b): every lane divides
+        // its determinant part by the ratio before the sums (an infinite or NaN sum when
+        // rho = 0 is never used: such a move is rejected)
+        const R irho = rcp_r<R>(R(rho));
+        R dn0 = fma(gn0, irho, jn0), dn1 = fma(gn1, irho, jn1), dn2 = fma(gn2, irho, jn2);
         if constexpr (sizeof(R) == 4) {
-            wave_sum2(gn0, gn1);
-            wave_sum2(gn2, jn0);
-            wave_sum2(jn1, jn2);
-            dU = wave_sum(dU);
+            wave_sum2(dn0, dn1);
+            wave_sum2(dn2, dU);
         } else {
-            gn0 = wave_sum(gn0);
-            gn1 = wave_sum(gn1);
-            gn2 = wave_sum(gn2);
-            jn0 = wave_sum(jn0);
-            jn1 = wave_sum(jn1);
-            jn2 = wave_sum(jn2);
+            dn0 = wave_sum(dn0);
+            dn1 = wave_sum(dn1);
+            dn2 = wave_sum(dn2);
             dU = wave_sum(dU);
         }
-        const double gn[3] = {double(gn0), double(gn1), double(gn2)};
-        const double jn[3] = {double(jn0), double(jn1), double(jn2)};
-        const double dlogj = double(dU);
+        const R dn[3] = {dn0, dn1, dn2};
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
         if (accept) {
             // Metropolis exponent of the drift-diffusion Green's functions: FP64 moves in FP64,
             // mixed moves in FP32 (its inputs carry FP32 errors already)
             if constexpr (sizeof(R) == 8) {
-                const double irho = 1.0 / rho;
                 double lq = 0.0;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    double drn = (gn[d] * irho + jn[d]) * sig2;
-                    if (S.cutdrift) drn = fmin(S.dcut, fmax(-S.dcut, drn));
+                    const double drn = fmin(dcut, fmax(-dcut, dn[d] * sig2));
                     lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
                 }
-                const double logp = 2.0 * (log(fabs(rho)) + dlogj) + lq * half_isig2;
+                const double logp = 2.0 * (log(fabs(rho)) + dU) + lq * half_isig2;
                 accept = u4 < exp(fmin(0.0, logp));
             } else {
-                const float irho = rcp_r<float>(float(rho)), fs2 = float(sig2), fcut = float(S.dcut);
+                const float fs2 = float(sig2), fcut = float(dcut);
                 float lq = 0.0f;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    float drn = (float(gn[d]) * irho + float(jn[d])) * fs2;
-                    if (S.cutdrift) drn = fminf(fcut, fmaxf(-fcut, drn));
-                    const float dof = float(dro[d]);
-                    lq += (dof + drn) * (dof - drn - 2.0f * float(dxm[d]));
+                    const float drn = fminf(fcut, fmaxf(-fcut, dn[d] * fs2));
+                    lq += (dro[d] + drn) * (dro[d] - drn - 2.0f * dxm[d]);
                 }
-                const float logp = 2.0f * (log_r<float>(fabsf(float(rho))) + float(dlogj)) +
+                const float logp = 2.0f * (log_r<float>(fabsf(float(rho))) + dU) +
                                    lq * float(half_isig2);
                 accept = u4 < double(exp_r<float>(fminf(0.0f, logp)));
             }
         }
         if (accept) {
             ++nacc;
-            const R irr = rcp_r<R>(R(rho));
+            const R irr = irho;
             if (tid < NUP) fb[tid] = fnew;
             wave_sync();
             if (tid < NUP) {
