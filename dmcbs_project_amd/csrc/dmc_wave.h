@@ -21,6 +21,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Keeps v computed at this point: an empty asm statement that reads and writes it.  Without
+// it the compiler may sink a sum to a distant use and hold all of its addends live in between.
+__device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// 1/sqrt(x) for x > 0 in the normal range: the hardware estimate refined by two Newton
+// steps y <- y + y (1/2 - x y^2 / 2), each roughly squaring the relative error (a few ulp at
+// the end; 8 operations instead of a correctly rounded square root and a division)
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
+}
+
 __device__ __forceinline__ double wrapd(double x, const DevSys &S) {
     return x - S.el * round(x * S.eli);  // Fortran nint: half away from zero
 }
@@ -584,7 +600,9 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
                 dx[d] -= S.el * rint(dx[d] * S.eli);
                 r2 += dx[d] * dx[d];
             }
-            const double rr = sqrt(r2), rj = fmax(1.0e-14, rr);
+            // r and 1/r from one reciprocal square root, r floored at 1e-14 (coshpot.f90:167;
+            // the floor also keeps coincident particles finite in the beta lookup)
+            const double irj = rsqrt_nr(fmax(r2, 1.0e-28)), rj = r2 * irj, rr = rj;
             if (rad) rad->v[a][b] = S.alpha0 + beta_val(S, rr);
             if (rj < S.range) {
                 double w0, w1, w2, w3;
@@ -592,7 +610,6 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
                 const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
                 V += jv.x;
                 U += jv.y;
-                const double irj = 1.0 / rj;
                 const double t = jv.z * irj * S.invsqrtm;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -626,6 +643,7 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
             double v = 0.0;
 #pragma unroll
             for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
+            pin(v);
             J.j[spin][d] = v;
         }
     J.U = wave_sum(U);
@@ -691,12 +709,12 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
                     dx[d] -= S.el * rint(dx[d] * S.eli);
                     r2 += dx[d] * dx[d];
                 }
-                const double rr = sqrt(r2);
+                const double ir = rsqrt_nr(fmax(r2, 1.0e-28)), rr = r2 * ir;
                 if (!S.nobeta && rr <= S.rangeb) {
                     double w0, w1, w2, w3;
                     const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
                     const double4 bv = lookup4(S.bt, idx, w0, w1, w2, w3);
-                    const double t = sv[a][b] * bv.y * (1.0 / rr);
+                    const double t = sv[a][b] * bv.y * ir;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         pu[a][d] += t * dx[d];
