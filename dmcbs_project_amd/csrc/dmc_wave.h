@@ -426,7 +426,10 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
 #pragma unroll
         for (int a = 0; a < TR; ++a)
 #pragma unroll
-            for (int b = 0; b < TR; ++b) Sm[a][b] += zi[a].x * zj[b].x + zi[a].y * zj[b].y;
+            for (int b = 0; b < TR; ++b) {
+                Sm[a][b] = fma(zi[a].x, zj[b].x, Sm[a][b]);
+                Sm[a][b] = fma(zi[a].y, zj[b].y, Sm[a][b]);
+            }
     }
     wave_sync();  // the lean layout puts the elimination scratch over the phases
 
@@ -757,8 +760,7 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
 #pragma unroll
         for (int d = 0; d < 3; ++d) gu[a][d] = gd[a][d] = 0.0;
     // plane-wave part: Sum_j Sinv(j, i) sin(q.r_ij) = s_i X_i - c_i Y_i with X = C^T Sinv
-    // for the up particles, and the same with Z, W for the down particles, in two passes
-    // over q (one set of accumulators live at a time)
+    // for the up particles, and the same with Z, W for the down particles, one pass over q
 #pragma unroll 1
     for (int q = 0; q < nq; ++q) {
         const double *cq = qc + 5 * q;
@@ -774,26 +776,17 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
             double X = 0.0, Y = 0.0;
 #pragma unroll
             for (int b = 0; b < TR; ++b) {
-                X += sv[a][b] * zj[b].x;
-                Y += sv[a][b] * zj[b].y;
+                X = fma(sv[a][b], zj[b].x, X);
+                Y = fma(sv[a][b], zj[b].y, Y);
             }
             const double t = zi[a].y * X - zi[a].x * Y;
             gu[a][0] += cq[1] * t;
             gu[a][1] += cq[2] * t;
             gu[a][2] += cq[3] * t;
-            ls += zi[a].x * X + zi[a].y * Y;
+            ls = fma(zi[a].x, X, ls);
+            ls = fma(zi[a].y, Y, ls);
         }
         lap += cq[4] * ls;
-    }
-#pragma unroll 1
-    for (int q = 0; q < nq; ++q) {
-        const double *cq = qc + 5 * q;
-        double2 zi[TR], zj[TR];
-#pragma unroll
-        for (int a = 0; a < TR; ++a) {
-            zi[a] = ph[q * NP + ri[a]];
-            zj[a] = ph[q * NP + NUP + cj[a]];
-        }
 #pragma unroll
         for (int b = 0; b < TR; ++b) {
             double Z = 0.0, W = 0.0;
