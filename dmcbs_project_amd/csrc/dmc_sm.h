@@ -86,11 +86,7 @@ __device__ __forceinline__ void sincos_2pi(double t, R &s, R &c) {
 __device__ __forceinline__ float lane_value(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
-__device__ __forceinline__ double lane_value(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    return dbl_of(__builtin_amdgcn_readlane(int(b & 0xffffffffll), l),
-                  __builtin_amdgcn_readlane(int(b >> 32), l));
-}
+__device__ __forceinline__ double lane_value(double v, int l) { return readlane_d(v, l); }
 __device__ __forceinline__ uint64_t lane_value(uint64_t v, int l) {
     const unsigned lo = __builtin_amdgcn_readlane(unsigned(v & 0xffffffffull), l);
     const unsigned hi = __builtin_amdgcn_readlane(unsigned(v >> 32), l);
