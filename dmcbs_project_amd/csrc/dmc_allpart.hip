@@ -31,7 +31,7 @@ __global__ __launch_bounds__(64, 2) void k_eval(DevSys S, const double *__restri
     PairTile<NUP> rad;
     const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid, &rad);
     const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid, &rad);
-    if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_bytes(S), tid);
+    if (sinv) store_sinv<NUP>(S, sm, L, D.singular, sinv + size_t(w) * sinv_stride(S), tid);
     grad_lap<NUP>(S, sm, L, T, D, J, tid);
     for (int t = tid; t < M; t += 64) dpsi[size_t(w) * M + t] = sm[L.dp + t];
     if (tid == 0) {

@@ -111,6 +111,17 @@ __global__ void k_branch_map(DevState src, DevStep T, int capacity, const int *g
     }
 }
 
+// copy of one walker's slot of nbytes (a multiple of 16, both ends 16-byte aligned: positions
+// are 24 N bytes with N even, inverse slots whole 128-byte lines) by the 64 lanes of a wave,
+// in 16-byte vectors with several loads in flight per lane
+__device__ __forceinline__ void copy_slot(const void *src, void *dst, size_t nbytes, int tid) {
+    const uint4 *a = static_cast<const uint4 *>(src);
+    uint4 *b = static_cast<uint4 *>(dst);
+    const int n = int(nbytes / 16);
+#pragma unroll 4
+    for (int t = tid; t < n; t += 64) b[t] = a[t];
+}
+
 // copy every destination walker from its source; weight becomes sign(1, weight)
 __global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState dst, DevStep T,
                                                int m3, const int *gate, int capacity) {
@@ -118,17 +129,11 @@ __global__ __launch_bounds__(64) void k_gather(DevSys S, DevState src, DevState 
     const int d = blockIdx.x;
     if (d >= T.nt[1] || d >= capacity) return;
     const int w = T.map[d];
-    const double *xs = src.x + size_t(w) * m3, *gs = src.dpsi + size_t(w) * m3;
-    double *xd = dst.x + size_t(d) * m3, *gd = dst.dpsi + size_t(d) * m3;
-    for (int t = threadIdx.x; t < m3; t += 64) {
-        xd[t] = xs[t];
-        gd[t] = gs[t];
-    }
+    copy_slot(src.x + size_t(w) * m3, dst.x + size_t(d) * m3, size_t(m3) * 8, threadIdx.x);
+    copy_slot(src.dpsi + size_t(w) * m3, dst.dpsi + size_t(d) * m3, size_t(m3) * 8, threadIdx.x);
     if (S.keep_sinv) {
-        const size_t nb = sinv_bytes(S) / 4;
-        const uint32_t *a = reinterpret_cast<const uint32_t *>(src.sinv + size_t(w) * nb * 4);
-        uint32_t *b = reinterpret_cast<uint32_t *>(dst.sinv + size_t(d) * nb * 4);
-        for (size_t t = threadIdx.x; t < nb; t += 64) b[t] = a[t];
+        const size_t sb = sinv_stride(S);
+        copy_slot(src.sinv + size_t(w) * sb, dst.sinv + size_t(d) * sb, sb, threadIdx.x);
     }
     if (threadIdx.x == 0) {
         dst.psil[d] = src.psil[w];
@@ -155,6 +160,8 @@ __global__ __launch_bounds__(256) void k_est_reduce(DevStep T, int nw, double *a
     const int g = tid / 42, k = tid - 42 * g;
     if (g < 6) {
         double s = 0.0;
+        // unrolled: several loads in flight, the sum still taken in walker order
+#pragma unroll 8
         for (int w = w0 + g; w < w1; w += 6) s += T.est[size_t(w) * 42 + k];
         part[g][k] = s;
     }
@@ -249,17 +256,11 @@ __global__ __launch_bounds__(64) void k_copy_back(DevSys S, DevState src, DevSta
     if (gated_off(gate)) return;
     const int d = blockIdx.x;
     if (d >= T.nt[1] || d >= capacity) return;
-    const double *xs = src.x + size_t(d) * m3, *gs = src.dpsi + size_t(d) * m3;
-    double *xd = dst.x + size_t(d) * m3, *gd = dst.dpsi + size_t(d) * m3;
-    for (int t = threadIdx.x; t < m3; t += 64) {
-        xd[t] = xs[t];
-        gd[t] = gs[t];
-    }
+    copy_slot(src.x + size_t(d) * m3, dst.x + size_t(d) * m3, size_t(m3) * 8, threadIdx.x);
+    copy_slot(src.dpsi + size_t(d) * m3, dst.dpsi + size_t(d) * m3, size_t(m3) * 8, threadIdx.x);
     if (S.keep_sinv) {
-        const size_t nb = sinv_bytes(S) / 4;
-        const uint32_t *a = reinterpret_cast<const uint32_t *>(src.sinv + size_t(d) * nb * 4);
-        uint32_t *b = reinterpret_cast<uint32_t *>(dst.sinv + size_t(d) * nb * 4);
-        for (size_t t = threadIdx.x; t < nb; t += 64) b[t] = a[t];
+        const size_t sb = sinv_stride(S);
+        copy_slot(src.sinv + size_t(d) * sb, dst.sinv + size_t(d) * sb, sb, threadIdx.x);
     }
     if (threadIdx.x == 0) {
         dst.psil[d] = src.psil[d];

@@ -89,7 +89,7 @@ struct dmc_ctx {
     }
     int alloc_state(DevState &S) {
         const size_t m = size_t(capacity) * 3 * sys.npart;
-        const size_t sb = ds.keep_sinv ? size_t(capacity) * sinv_bytes(ds) : 1;
+        const size_t sb = ds.keep_sinv ? size_t(capacity) * sinv_stride(ds) : 1;
         if (alloc(&S.x, m) || alloc(&S.dpsi, m) || alloc(&S.psil, capacity) ||
             alloc(&S.d2psi, capacity) || alloc(&S.v, capacity) || alloc(&S.weight, capacity) ||
             alloc(&S.wt0, capacity) || alloc(&S.is, capacity) || alloc(&S.irn, capacity) ||
@@ -1008,7 +1008,7 @@ int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out)
             dmc_ctx *c = b->ctx[b->order[i]];
             struct { void *p; size_t n; } r[3] = {
                 {c->T.moves, 2 * sizeof(int64_t)}, {c->T.hist, 500 * sizeof(int64_t)},
-                {c->A.sinv, size_t(c->nw) * sinv_bytes(c->ds)}};
+                {c->A.sinv, size_t(c->nw) * sinv_stride(c->ds)}};
             for (int k = 0; k < 3; ++k) {
                 auto &v = save[3 * i + k];
                 if (!restore) v.resize(r[k].n);

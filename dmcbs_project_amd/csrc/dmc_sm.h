@@ -351,7 +351,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
     bool colmode = m0 < NUP;
     R Ac[NUP];
     {
-        const R *g = reinterpret_cast<const R *>(sinv_g + size_t(w) * sinv_bytes(S));
+        const R *g = reinterpret_cast<const R *>(sinv_g + size_t(w) * sinv_stride(S));
         const int k = tid < NUP ? tid : 0;
 #pragma unroll
         for (int b = 0; b < NUP; ++b) Ac[b] = g[colmode ? b * NUP + k : k * NUP + b];
@@ -675,7 +675,7 @@ __device__ __forceinline__ void sm_refresh_walker(const DevSys &S, double *sm, i
     wave_sync();
     const Tile<NUP> Tl(tid);
     const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid);
-    store_sinv<NUP>(S, sm, L, D.singular, sinv_g + size_t(w) * sinv_bytes(S), tid);
+    store_sinv<NUP>(S, sm, L, D.singular, sinv_g + size_t(w) * sinv_stride(S), tid);
 }
 
 // end of the sweep: FP64 hpsi of the final positions B.x, the fresh inverse into B.sinv,
@@ -694,7 +694,7 @@ __device__ __forceinline__ void sm_end_walker(const DevSys &S, double *sm, int w
     PairTile<NUP> rad;
     const JSums J = jastrow_terms<NUP>(S, sm, L, Tl, tid, &rad);
     const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid, &rad);
-    store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_bytes(S), tid);
+    store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_stride(S), tid);
     grad_lap<NUP>(S, sm, L, Tl, D, J, tid);
     finish_walker<NUP>(S, sm, L, A, B, T, w, tid, B.irn[w], opcomp);
 }

@@ -905,6 +905,12 @@ __host__ __device__ inline size_t sinv_bytes(const DevSys &S) {
     return size_t(S.n) * S.n * (S.sinv_f32 ? 4 : 8);
 }
 
+// per-walker slot of the stored inverse: the dense matrix padded to whole 128-byte lines (the
+// branch copies slots with 16-byte vector loads)
+__host__ __device__ inline size_t sinv_stride(const DevSys &S) {
+    return (sinv_bytes(S) + 127) / 128 * 128;
+}
+
 template <int NUP>
 __device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
                                            int tid) {
