@@ -4,6 +4,8 @@
 #pragma once
 
 
+#include <type_traits>
+
 #include "dmc_kernels.h"
 
 namespace dmc {
@@ -19,6 +21,15 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// f(std::integral_constant<int, I>) for I = B .. E-1 (a loop over a compile-time index)
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
 }
 
 // Keeps v computed at this point: an empty asm statement that reads and writes it.  Without
@@ -460,105 +471,96 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
     double prod = 1.0;
     int expo = 0, sgn = 1;
     bool singular = false;
-    // One pivot per iteration, the loop kept rolled: a compact body whose registers are
-    // those of one step.  Tile column bk of the pivot column and tile row ap of the pivot
-    // row are wave-uniform, picked by scalar branches.
+    // The pivots in blocks of eight, one per tile column BK (a compile-time index, so the
+    // pivot column is read and overwritten in place), one pivot per iteration of a rolled
+    // loop within a block (a compact body whose registers are those of one step).  Tile row
+    // ap of the pivot row is wave-uniform, picked by scalar branches.
+    auto pivot_block = [&](auto bkc) __attribute__((always_inline)) {
+        constexpr int BK = decltype(bkc)::value;
+        constexpr int KN = NUP - 8 * BK < 8 ? NUP - 8 * BK : 8;
 #pragma unroll 1
-    for (int k = 0; k < NUP; ++k) {
-        const int bk = k >> 3, ck = k & 7;
-        double colk[TR];
+        for (int ck = 0; ck < KN; ++ck) {
+            const int k = 8 * BK + ck;
+            double best = -1.0;
+            int brow = 1 << 20;
+            if (c == ck) {
 #pragma unroll
-        for (int a = 0; a < TR; ++a) colk[a] = Sm[a][0];
-#pragma unroll
-        for (int b = 1; b < TR; ++b)
-            if (b == bk) {
-#pragma unroll
-                for (int a = 0; a < TR; ++a) colk[a] = Sm[a][b];
-            }
-        double best = -1.0;
-        int brow = 1 << 20;
-        if (c == ck) {
-#pragma unroll
-            for (int a = 0; a < TR; ++a) {
-                const double v = fabs(colk[a]);
-                if (!((used >> a) & 1u) && v > best) {
-                    best = v;
-                    brow = r + 8 * a;
+                for (int a = 0; a < TR; ++a) {
+                    const double v = fabs(Sm[a][BK]);
+                    if (!((used >> a) & 1u) && v > best) {
+                        best = v;
+                        brow = r + 8 * a;
+                    }
                 }
             }
-        }
-        pivot_argmax(best, brow, tid);
-        const int p = __builtin_amdgcn_readlane(brow, ck);
-        const int ap = p >> 3;
-        // the pivot from the lane that holds it (tile row ap of a lane of column group ck)
-        // and its reciprocal, formed while the pivot row and column go through LDS
-        double pe = colk[0];
-#pragma unroll
-        for (int a = 1; a < TR; ++a)
-            if (a == ap) pe = colk[a];
-        const double piv = readlane_d(pe, 8 * (p & 7) + ck);
-        const double rp = rcp_nr(piv);
-        if (r == (p & 7)) {
-            double rowp[TR];
-#pragma unroll
-            for (int b = 0; b < TR; ++b) rowp[b] = Sm[0][b];
+            pivot_argmax(best, brow, tid);
+            const int p = __builtin_amdgcn_readlane(brow, ck);
+            const int ap = p >> 3;
+            // the pivot from the lane that holds it (tile row ap of a lane of column group
+            // ck) and its reciprocal, formed while the pivot row and column go through LDS
+            double pe = Sm[0][BK];
 #pragma unroll
             for (int a = 1; a < TR; ++a)
+                if (a == ap) pe = Sm[a][BK];
+            const double piv = readlane_d(pe, 8 * (p & 7) + ck);
+            const double rp = rcp_nr(piv);
+            if (r == (p & 7)) {
+                double rowp[TR];
+#pragma unroll
+                for (int b = 0; b < TR; ++b) rowp[b] = Sm[0][b];
+#pragma unroll
+                for (int a = 1; a < TR; ++a)
+                    if (a == ap) {
+#pragma unroll
+                        for (int b = 0; b < TR; ++b) rowp[b] = Sm[a][b];
+                    }
+#pragma unroll
+                for (int b = 0; b < TR; ++b) prow[c + 8 * b] = rowp[b];
+            }
+            if (c == ck) {
+#pragma unroll
+                for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][BK] * rp;  // multipliers
+            }
+            if (tid == 0) {
+                perm[k] = p;
+                pinv[p] = k;
+            }
+            wave_sync();
+            singular |= (piv == 0.0);
+            sgn = piv < 0.0 ? -sgn : sgn;
+            prod *= fabs(piv);
+            double f[TR], pv[TR];
+#pragma unroll
+            for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a];
+#pragma unroll
+            for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
+            // elimination of every element, then the pivot column (lanes c == ck, tile
+            // column BK) and the pivot row (lanes r == p & 7, tile row ap) overwritten
+#pragma unroll
+            for (int a = 0; a < TR; ++a)
+#pragma unroll
+                for (int b = 0; b < TR; ++b) Sm[a][b] -= f[a] * pv[b];
+            const bool colk_lane = c == ck, prow_lane = r == (p & 7);
+#pragma unroll
+            for (int a = 0; a < TR; ++a) Sm[a][BK] = colk_lane ? -f[a] : Sm[a][BK];
+#pragma unroll
+            for (int a = 0; a < TR; ++a)
                 if (a == ap) {
 #pragma unroll
-                    for (int b = 0; b < TR; ++b) rowp[b] = Sm[a][b];
+                    for (int b = 0; b < TR; ++b) {
+                        const double prw = (b == BK && colk_lane) ? rp : pv[b] * rp;
+                        Sm[a][b] = prow_lane ? prw : Sm[a][b];
+                    }
+                    if (prow_lane) used |= 1u << a;
                 }
-#pragma unroll
-            for (int b = 0; b < TR; ++b) prow[c + 8 * b] = rowp[b];
+            wave_sync();
         }
-        if (c == ck) {
-#pragma unroll
-            for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = colk[a] * rp;  // the multipliers
-        }
-        if (tid == 0) {
-            perm[k] = p;
-            pinv[p] = k;
-        }
-        wave_sync();
-        singular |= (piv == 0.0);
-        sgn = piv < 0.0 ? -sgn : sgn;
-        prod *= fabs(piv);
-        if ((k & 7) == 7) {
-            int e;
-            prod = frexp(prod, &e);
-            expo += e;
-        }
-        double f[TR], pv[TR];
-#pragma unroll
-        for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a];
-#pragma unroll
-        for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
-        // elimination of every element, then the pivot column (lanes c == ck, tile column
-        // bk) and the pivot row (lanes r == p & 7, tile row ap) overwritten: bk and ap are
-        // wave-uniform, so only the selects of one tile column and one tile row remain
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < TR; ++b) Sm[a][b] -= f[a] * pv[b];
-        const bool colk_lane = c == ck, prow_lane = r == (p & 7);
-#pragma unroll
-        for (int b = 0; b < TR; ++b)
-            if (b == bk) {
-#pragma unroll
-                for (int a = 0; a < TR; ++a) Sm[a][b] = colk_lane ? -f[a] : Sm[a][b];
-            }
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-            if (a == ap) {
-#pragma unroll
-                for (int b = 0; b < TR; ++b) {
-                    const double prw = (b == bk && colk_lane) ? rp : pv[b] * rp;
-                    Sm[a][b] = prow_lane ? prw : Sm[a][b];
-                }
-                if (prow_lane) used |= 1u << a;
-            }
-        wave_sync();
-    }
+        // the running product of |pivots| renormalised after every block (exact)
+        int e;
+        prod = frexp(prod, &e);
+        expo += e;
+    };
+    static_for<0, TR>(pivot_block);
 
     // permutation parity: n - #cycles
     int ismin = 0;
