@@ -882,24 +882,27 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
 }
 
 // copy the inverse left in LDS by build_inverse to global memory, dense [j][i], as FP64
-// or FP32 (zeros for a singular matrix)
+// or FP32 (zeros for a singular matrix), in 16-byte stores
 template <int NUP>
 __device__ void store_sinv(const DevSys &S, const double *sm, const Layout<NUP> &L, bool singular,
                            char *dst, int tid) {
     constexpr int SP = Layout<NUP>::SP;
     const double *sv = sm + L.sinv;
+    constexpr int NE = NUP * NUP;
+    // element e of the dense slot, zero beyond the matrix (the slot is padded to whole
+    // 128-byte lines, so the 16-byte groups below stay inside it)
+    auto elem = [&](int e) __attribute__((always_inline)) {
+        const int j = e / NUP, i = e - j * NUP;
+        return (e < NE && !singular) ? sv[j * SP + i] : 0.0;
+    };
     if (S.sinv_f32) {
-        float *d = reinterpret_cast<float *>(dst);
-        for (int e = tid; e < NUP * NUP; e += 64) {
-            const int j = e / NUP, i = e - j * NUP;
-            d[e] = singular ? 0.0f : float(sv[j * SP + i]);
-        }
+        float4 *d = reinterpret_cast<float4 *>(dst);
+        for (int g = tid; g < (NE + 3) / 4; g += 64)
+            d[g] = make_float4(float(elem(4 * g)), float(elem(4 * g + 1)), float(elem(4 * g + 2)),
+                               float(elem(4 * g + 3)));
     } else {
-        double *d = reinterpret_cast<double *>(dst);
-        for (int e = tid; e < NUP * NUP; e += 64) {
-            const int j = e / NUP, i = e - j * NUP;
-            d[e] = singular ? 0.0 : sv[j * SP + i];
-        }
+        double2 *d = reinterpret_cast<double2 *>(dst);
+        for (int g = tid; g < (NE + 1) / 2; g += 64) d[g] = make_double2(elem(2 * g), elem(2 * g + 1));
     }
 }
 
