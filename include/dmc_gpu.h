@@ -33,7 +33,8 @@ enum {
 /* precision of the propagator */
 enum { DMC_FP64 = 0, DMC_MIXED = 1 };
 /* move type: the reference's all-particle DMC move (step.f90:56-76), single-particle DMC
- * moves with Sherman-Morrison inverse updates and an FP64 LU refresh every krefresh moves, or
+ * moves with Sherman-Morrison inverse updates and an FP64 rebuild of the inverse (Gauss-Jordan
+ * elimination with partial pivoting) every krefresh moves, or
  * the reference's variational Metropolis step (step1 with idmc = 0, step.f90:78-122: no
  * weights, no branching; estimators 1-6 every step) */
 enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1, DMC_VMC = 2 };
