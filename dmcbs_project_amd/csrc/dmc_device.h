@@ -16,6 +16,10 @@ struct DevSys {
         nfac, twopi_l, popup, popdn;
     const double4 *jt;    // Jastrow/potential records {v, u, du, d2u}, [ntab+1]
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
+    // the same beta values in compact arrays, so that a pair loop touches only the fields it
+    // reads (a smaller working set in the vector L1): b, [ntab+1]; {db, lap b}, [ntab+1]
+    const double *bt1;
+    const double2 *bt2;
     const double4 *mt64;  // pair records of the single-particle moves {b, db, u, du}, [ntab+1]
     const float4 *mt32;   // and their FP32 copy (mixed-precision moves)
     const double4 *btb;   // d/db and d/dd of the beta records (parameter adjoint)

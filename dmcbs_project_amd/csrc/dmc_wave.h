@@ -99,7 +99,7 @@ __device__ __forceinline__ double beta_val(const DevSys &S, double r) {
     if (S.nobeta || r > S.rangeb) return 0.0;
     double w0, w1, w2, w3;
     const int idx = lagrange(S.scaleb * r, S.ntab, w0, w1, w2, w3);
-    return w0 * S.bt[idx - 1].x + w1 * S.bt[idx].x + w2 * S.bt[idx + 1].x + w3 * S.bt[idx + 2].x;
+    return w0 * S.bt1[idx - 1] + w1 * S.bt1[idx] + w2 * S.bt1[idx + 1] + w3 * S.bt1[idx + 2];
 }
 
 // Sum over the 64 lanes of a wave, returned as a wave-uniform value.  Within each 16-lane
@@ -742,14 +742,17 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
                 if (!S.nobeta && rr <= S.rangeb) {
                     double w0, w1, w2, w3;
                     const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
-                    const double4 bv = lookup4(S.bt, idx, w0, w1, w2, w3);
-                    const double t = sv[a][b] * bv.y * ir;
+                    const double2 A = S.bt2[idx - 1], B = S.bt2[idx], C = S.bt2[idx + 1],
+                                  D = S.bt2[idx + 2];
+                    const double2 bv = make_double2(w0 * A.x + w1 * B.x + w2 * C.x + w3 * D.x,
+                                                    w0 * A.y + w1 * B.y + w2 * C.y + w3 * D.y);
+                    const double t = sv[a][b] * bv.x * ir;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         pu[a][d] += t * dx[d];
                         pdn[b][d] -= t * dx[d];
                     }
-                    lap += sv[a][b] * bv.z;
+                    lap += sv[a][b] * bv.y;
                 }
                 __builtin_amdgcn_sched_barrier(0);  // one pair's table loads in flight
             }
