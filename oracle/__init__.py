@@ -81,8 +81,55 @@ def lib():
         L.dmco_sm_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
                                     C.c_int32, _f64p, _f64p, _f64p, _i64p, _f64p, _i32p,
                                     C.c_int32]
+        L.dmco_kshells.argtypes = [C.c_double, C.c_int32, _i32p, _f64p, C.c_int32]
+        L.dmco_kshells.restype = C.c_int
+        L.dmco_addgofr.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, _f64p,
+                                   C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.dmco_addsofq.argtypes = [C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
+                                   _f64p, _f64p, _f64p]
         _lib = L
     return _lib
+
+
+def kshells(el, nsh, maxk=4096):
+    """setupk (kshell.f90:6-156): (numsh[nsh], ak[nk][3]) of the first nsh shells"""
+    numsh = np.zeros(nsh, np.int32)
+    ak = np.zeros(3 * maxk)
+    nk = lib().dmco_kshells(float(el), int(nsh), numsh, ak, int(maxk))
+    if nk < 0:
+        raise ValueError(f"setupk failed ({nk})")
+    return numsh, ak[: 3 * nk].reshape(nk, 3).copy()
+
+
+def addgofr(configs, weights, el, ntab, nup=None):
+    """addgofr (gofr.f90:52-98) over configurations [nc][npart][3] with their weights:
+    the accumulators gnow, gerr [ntab+1][3], dnow, derr [2 ntab+1][6] and wsblk"""
+    configs = np.ascontiguousarray(configs, np.float64)
+    npart = configs.shape[1]
+    nup = npart // 2 if nup is None else nup
+    g, ge = np.zeros((ntab + 1) * 3), np.zeros((ntab + 1) * 3)
+    d, de = np.zeros((2 * ntab + 1) * 6), np.zeros((2 * ntab + 1) * 6)
+    ws = np.zeros(1)
+    for x, w in zip(configs, weights):
+        lib().dmco_addgofr(npart, nup, float(el), int(ntab), np.ascontiguousarray(x).ravel(),
+                           float(w), g, ge, d, de, ws)
+    return (g.reshape(-1, 3), ge.reshape(-1, 3), d.reshape(-1, 6), de.reshape(-1, 6),
+            float(ws[0]))
+
+
+def addsofq(configs, weights, ak, nup=None):
+    """addsofq (sofq.f90:40-65) over configurations with their weights: snow, serr
+    [nq][3] and wsblk"""
+    configs = np.ascontiguousarray(configs, np.float64)
+    npart = configs.shape[1]
+    nup = npart // 2 if nup is None else nup
+    ak = np.ascontiguousarray(ak, np.float64)
+    nq = ak.shape[0]
+    sn, se, ws = np.zeros(3 * nq), np.zeros(3 * nq), np.zeros(1)
+    for x, w in zip(configs, weights):
+        lib().dmco_addsofq(npart, nup, nq, ak.ravel(), np.ascontiguousarray(x).ravel(),
+                           float(w), sn, se, ws)
+    return sn.reshape(nq, 3), se.reshape(nq, 3), float(ws[0])
 
 
 DEFAULTS = dict(npart=14, norb=3, ntab=1024, fn=1, units=1, nwalk=1000, rho=0.0337737278,

@@ -38,7 +38,8 @@ struct dmco_ctx {
     double *bt, *dbt, *d2bt;
     int nobeta;
     /* k vectors */
-    int nk, numsh[DMCO_MAXORB];
+    int nk;
+    int32_t numsh[DMCO_MAXORB];
     double ak[MAXK][3], ak2[MAXK];
     double vorb[DMCO_MAXORB];
     double pi;
@@ -84,8 +85,11 @@ static inline double wrap(double x, double el, double eli) { return x - el * rou
 static inline double wrapd(double x, double el) { return x - el * round(x / el); }
 
 /* ---------------------------------------------------------------- k shells */
-/* kshell.f90:20-156 kstates3d */
-static int kstates3d(dmco_ctx *c, int nsh0) {
+/* kshell.f90:20-156 kstates3d: the first nsh0 shells of half-space k-vectors, ak = 2 pi m / L;
+ * numsh0[s] vectors in shell s.  Returns the number of vectors, or -1 ("Too many states")
+ * or -2 (more than maxk vectors). */
+static int kshell_core(double el, int nsh0, int32_t *numsh0, double (*akv)[3], double *ak2v,
+                       int maxk) {
     int i2max = 2 * nsh0;
     int imax = (int)lroundf(sqrtf((float)i2max));
     int cap = (imax + 1) * (imax + 1) * (imax + 1) * 4 + 8;
@@ -123,21 +127,39 @@ static int kstates3d(dmco_ctx *c, int nsh0) {
     }
     for (int i = 1; i <= nsh; i++) numsh[i] *= 2;
     if (nsh0 >= nsh) { free(ik); free(numsh); return -1; } /* "Too many states" */
-    double piel = 2.0 * c->pi / c->el;
+    double piel = 2.0 * (4.0 * atan(1.0)) / el;
     int k = 0;
     for (int i = 1; i <= nsh0; i++) {
-        c->numsh[i - 1] = numsh[i] / 2;
+        numsh0[i - 1] = numsh[i] / 2;
         int m = numsh[i] / 2 > 1 ? numsh[i] / 2 : 1;
         for (int j = 0; j < m; j++) {
-            for (int d = 0; d < 3; d++) c->ak[k][d] = piel * ik[k][d];
-            c->ak2[k] = c->ak[k][0] * c->ak[k][0] + c->ak[k][1] * c->ak[k][1] + c->ak[k][2] * c->ak[k][2];
+            if (k >= maxk) { free(ik); free(numsh); return -2; }
+            for (int d = 0; d < 3; d++) akv[k][d] = piel * ik[k][d];
+            ak2v[k] = akv[k][0] * akv[k][0] + akv[k][1] * akv[k][1] + akv[k][2] * akv[k][2];
             k++;
         }
     }
-    c->nk = k;
     free(ik);
     free(numsh);
+    return k;
+}
+
+static int kstates3d(dmco_ctx *c, int nsh0) {
+    int k = kshell_core(c->el, nsh0, c->numsh, c->ak, c->ak2, MAXK);
+    if (k < 0) return -1;
+    c->nk = k;
     return 0;
+}
+
+int dmco_kshells(double el, int32_t nsh0, int32_t *numsh, double *ak, int32_t maxk) {
+    double (*akv)[3] = malloc(sizeof(double[3]) * (maxk > 0 ? maxk : 1));
+    double *ak2v = malloc(sizeof(double) * (maxk > 0 ? maxk : 1));
+    int k = kshell_core(el, nsh0, numsh, akv, ak2v, maxk);
+    for (int i = 0; i < k; i++)
+        for (int d = 0; d < 3; d++) ak[3 * i + d] = akv[i][d];
+    free(akv);
+    free(ak2v);
+    return k;
 }
 
 /* ---------------------------------------------------------------- beta(r) */
@@ -1193,4 +1215,77 @@ void dmco_sm_step1(const dmco_ctx *c, int32_t nw, const double *x, const double 
         sm_one(c, x + o, dpsi + o, sc + 6 * w, irn[w], krefresh, x_out + o, dpsi_out + o,
                sc_out + 6 * w, irn_out + w, wt_out + w, nacc_out + w);
     }
+}
+
+/* ---------------------------------------------------------------- observables */
+/* gofr.f90:52-98 addgofr with ndim = 3: pair-distance histogram by spin channel and the
+ * one-body density profile along each axis, of one configuration x[npart][3] (up spins
+ * 0..nup-1).  Layouts follow the reference's column-major arrays: gnow/gerr [ntab+1][3]
+ * (channel 0 up-down, 1 up-up, 2 down-down), dnow/derr [2 ntab+1][6] (bin b is index
+ * b - ntab; channels x, y, z of spin up, then of spin down).  dr = L/(2 ntab)
+ * (gofr.f90:35-36). */
+void dmco_addgofr(int32_t npart, int32_t nup, double el, int32_t ntab, const double *x,
+                  double weight, double *gnow, double *gerr, double *dnow, double *derr,
+                  double *wsblk) {
+    double dr = 0.5 * el / ntab, scale = 1.0 / dr;
+    for (int i = 0; i < npart - 1; i++)
+        for (int j = i + 1; j < npart; j++) {
+            int itab = 1;
+            if (i < nup && j < nup) itab = 2;
+            if (i >= nup && j >= nup) itab = 3;
+            double dx[3], r2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+                dx[d] = x[3 * i + d] - x[3 * j + d];
+                dx[d] = dx[d] - el * round(dx[d] / el);
+            }
+            for (int d = 0; d < 3; d++) r2 += dx[d] * dx[d];
+            double r = sqrt(r2);
+            int index = r < 0.5 * el ? (int)(scale * r) : ntab;
+            gnow[3 * index + itab - 1] += 2.0 * weight;
+            gerr[3 * index + itab - 1] += 4.0 * weight;
+        }
+    *wsblk += weight;
+    for (int i = 0; i < npart; i++) {
+        for (int is = 0; is < 3; is++) {
+            double dx = x[3 * i + is] - 0.0;
+            dx = dx - el * round(dx / el);
+            int itab = i < nup ? is : is + 3;
+            int index = (int)(scale * dx);
+            int e = 6 * (index + ntab) + itab;
+            if (index == 0) {
+                dnow[e] += 0.5 * weight;
+                derr[e] += 0.25 * weight;
+            } else {
+                dnow[e] += weight;
+                derr[e] += weight;
+            }
+        }
+    }
+}
+
+/* sofq.f90:40-65 addsofq: rho_up(q), rho_dn(q) = sum exp(-i q.x) over each spin, and
+ * snow/serr [nq][3] += (|rho_up|^2, |rho_dn|^2, Re rho_up conj(rho_dn)) * weight and their
+ * squares (real parts) * weight; ak[nq][3] from setupk. */
+void dmco_addsofq(int32_t npart, int32_t nup, int32_t nq, const double *ak, const double *x,
+                  double weight, double *snow, double *serr, double *wsblk) {
+    for (int iq = 0; iq < nq; iq++) {
+        double ur = 0.0, ui = 0.0, dr_ = 0.0, di = 0.0;
+        for (int i = 0; i < npart; i++) {
+            double arg = ak[3 * iq] * x[3 * i] + ak[3 * iq + 1] * x[3 * i + 1] +
+                         ak[3 * iq + 2] * x[3 * i + 2];
+            double c = cos(arg), s = -sin(arg);
+            if (i < nup) { ur += c; ui += s; } else { dr_ += c; di += s; }
+        }
+        /* z1 = rho_up conj(rho_up), z2 = rho_dn conj(rho_dn), z3 = rho_up conj(rho_dn) */
+        double z1r = ur * ur + ui * ui, z1i = ui * ur - ur * ui;
+        double z2r = dr_ * dr_ + di * di, z2i = di * dr_ - dr_ * di;
+        double z3r = ur * dr_ + ui * di, z3i = ui * dr_ - ur * di;
+        snow[3 * iq] += z1r * weight;
+        serr[3 * iq] += (z1r * z1r - z1i * z1i) * weight;
+        snow[3 * iq + 1] += z2r * weight;
+        serr[3 * iq + 1] += (z2r * z2r - z2i * z2i) * weight;
+        snow[3 * iq + 2] += z3r * weight;
+        serr[3 * iq + 2] += (z3r * z3r - z3i * z3i) * weight;
+    }
+    *wsblk += weight;
 }

@@ -101,4 +101,17 @@ void dmco_sm_step1(const dmco_ctx *c, int32_t nw, const double *x, const double 
                    const double *sc, const int64_t *irn, int32_t krefresh, double *x_out,
                    double *dpsi_out, double *sc_out, int64_t *irn_out, double *wt_out,
                    int32_t *nacc_out, int32_t nthreads);
+
+/* Observables (the operators of operators.f90:92-108) on one configuration x[npart][3]. */
+/* setupk with ndim = 3 (kshell.f90:6-156) for nsh0 shells of box side el: numsh[nsh0],
+ * ak[k][3]; returns the number of vectors, -1 when nsh0 exceeds the enumerated shells,
+ * -2 when more than maxk vectors */
+int dmco_kshells(double el, int32_t nsh0, int32_t *numsh, double *ak, int32_t maxk);
+/* addgofr (gofr.f90:52-98): gnow/gerr [ntab+1][3], dnow/derr [2 ntab+1][6] */
+void dmco_addgofr(int32_t npart, int32_t nup, double el, int32_t ntab, const double *x,
+                  double weight, double *gnow, double *gerr, double *dnow, double *derr,
+                  double *wsblk);
+/* addsofq (sofq.f90:40-65): snow/serr [nq][3] */
+void dmco_addsofq(int32_t npart, int32_t nup, int32_t nq, const double *ak, const double *x,
+                  double weight, double *snow, double *serr, double *wsblk);
 #endif

@@ -19,6 +19,8 @@
 ! and then calls the reference's own setpsi/setphi/setstep/hpsi/step1/branch/
 ! move1/getderpsi/update/result unchanged.
 !
+! The observables mode also links the reference's gofr.f90 and sofq.f90.
+!
 ! Usage:  ref_driver <mode> <infile> <outfile>  < deck
 ! All binary files are little-endian stream files of int64 / float64 values.
 program ref_driver
@@ -33,6 +35,8 @@ program ref_driver
    use betafun
    use kshell
    use optimizer
+   use gofr
+   use sofq
    implicit none
    integer, parameter :: i4=selected_int_kind(9)
    integer, parameter :: i8=selected_int_kind(15)
@@ -123,6 +127,8 @@ program ref_driver
          call do_dmc(.true.)
       case ('opt')
          call do_opt
+      case ('obs')
+         call do_obs
       case default
          write (6,*) 'unknown mode ',trim(mode)
          stop 1
@@ -426,5 +432,35 @@ contains
    endif
    close (21)
    end subroutine do_dmc
+
+! g(r), the density profile and S(q) of configurations read with their weights from infile:
+! the reference's setupgofr/addgofr/updategofr/writegofr (gofr.f90:14-176) and
+! setupsofq/addsofq/updatesofq/writesofq (sofq.f90:14-118) unchanged, as operators.f90:66-67,
+! 103-104, 120-121 and 137-138 call them; the tables go to gofr.out, rho.out and sofq.out in
+! the working directory, outfile gets the number of configurations, el and k_F.
+   subroutine do_obs
+   integer(kind=i8) :: nc,k,ntg,ntq
+   real(kind=r8) :: wt
+   call openin
+   read (22) ntg
+   read (22) ntq
+   read (22) nc
+   call setupgofr(ndim,int(ntg,i4),nbin,el,kunit)
+   call setupsofq(ndim,nbin,el,int(ntq,i4),kunit)
+   do k=1,nc
+      call read_config(w1%x)
+      read (22) wt
+      call addgofr(w1%x,wt)
+      call addsofq(w1%x,wt)
+   enddo
+   close (22)
+   call updategofr
+   call updatesofq
+   call writegofr('gofr.out',0.0_r8)
+   call writesofq('sofq.out',0.0_r8)
+   call openout
+   write (21) real(nc,r8),el,kunit
+   close (21)
+   end subroutine do_obs
 
 end program ref_driver

@@ -219,6 +219,45 @@ def main_vmc():
     opt_fixture(66, 5, random_configs(66, 16, 6066), "N66")
 
 
+def _table(text):
+    """rows of numbers of one output table (comment and blank lines skipped)"""
+    rows = []
+    for line in text.splitlines():
+        t = line.strip()
+        if not t or t.startswith("#"):
+            continue
+        rows.append([float(v.replace("D", "E")) for v in t.split()])
+    return np.array(rows)
+
+
+def obs_fixture(n, norb, configs, weights, ntg, ntq, tag):
+    """g(r), density profile and S(q) of weighted configurations through the reference's
+    gofr.f90/sofq.f90 (setup, add per configuration, update, write), as the operators of
+    operators.f90 apply them at the end of a block"""
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    nc = len(configs)
+    body = [pack([ntg], [ntq], [nc])]
+    for x, w in zip(configs, weights):
+        body.append(pack(x, [w]))
+    data, _, texts = refrun.run("obs", deck, b"".join(body),
+                                files=("gofr.out", "rho.out", "sofq.out"))
+    r = Reader(data)
+    assert int(r.f()) == nc
+    el, kf = r.f(), r.f()
+    np.savez(os.path.join(HERE, f"obs_{tag}.npz"), npart=n, norb=norb, x=configs, w=weights,
+             el=el, kf=kf,
+             ntg=ntg, ntq=ntq, gofr=_table(texts["gofr.out"]), rho=_table(texts["rho.out"]),
+             sofq=_table(texts["sofq.out"]), deck=deck)
+
+
+def main_obs():
+    # uniform configurations with weights of both signs (is * wt0 = -1 walkers included)
+    for n, norb, nc, ntg, ntq, seed in ((14, 3, 24, 40, 6, 7014), (66, 5, 12, 60, 8, 7066)):
+        rng = np.random.default_rng(seed)
+        w = rng.uniform(0.5, 1.5, nc) * np.where(rng.random(nc) < 0.2, -1.0, 1.0)
+        obs_fixture(n, norb, random_configs(n, nc, seed), w, ntg, ntq, f"N{n}")
+
+
 def main():
     main_vmc()
     for n, norb in SYSTEMS.items():
@@ -248,4 +287,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main_vmc() if "--vmc" in sys.argv else main()
+    if "--vmc" in sys.argv:
+        main_vmc()
+    elif "--obs" in sys.argv:
+        main_obs()
+    else:
+        main()

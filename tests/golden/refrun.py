@@ -45,7 +45,9 @@ def pack(*items):
     return b"".join(out)
 
 
-def run(mode, deck, inbytes, timeout=3600):
+def run(mode, deck, inbytes, timeout=3600, files=()):
+    """Run the driver in a scratch directory; returns (outfile bytes, stdout) and, when files
+    names text files the driver writes there, also their contents."""
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
@@ -57,4 +59,11 @@ def run(mode, deck, inbytes, timeout=3600):
         if r.returncode != 0:
             raise RuntimeError(f"ref_driver {mode} failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
         with open(fout, "rb") as fh:
-            return fh.read(), r.stdout.decode(errors="replace")
+            data = fh.read()
+        if files:
+            texts = {}
+            for f in files:
+                with open(os.path.join(td, f)) as fh:
+                    texts[f] = fh.read()
+            return data, r.stdout.decode(errors="replace"), texts
+        return data, r.stdout.decode(errors="replace")
