@@ -8,7 +8,7 @@ The mixed-precision propagator (FP32 moves, FP64 refresh/energies/weights/sign t
 reported value; the plain FP64 propagator runs beside it for the ratio, and the per-step
 local-energy error of the mixed path against FP64 is measured on the same configurations.
 
-    python bench.py [--steps K] [--warmup W] [--workload mixed|fp64|allparticle]
+    python bench.py [--steps K] [--warmup W] [--workload mixed|fp64|allparticle|batch|observables]
 
 Prints one JSON line.  A "step" is one DMC step of the whole resident population; value is
 walker-steps per second (sum of the populations of the timed steps / wall time), with the
@@ -226,6 +226,29 @@ def batch_run(steps, warmup, tune=True):
     return res
 
 
+def observables_run(steps, warmup, ntab=100, ntabq=10):
+    """g(r) with density profile (addgofr, ntab bins) and S(q) (addsofq, ntabq shells) over
+    the resident headline population after 3 mixed-precision steps: one "step" is one block's
+    operator pass over every walker (dmcbcs.f90:465-471), host sums included."""
+    e = make_engine("mixed")
+    for _ in range(3):
+        e.step()
+    gacc, sacc = e.addgofr(ntab), e.addsofq(ntabq)
+    for _ in range(warmup):
+        e.addgofr(ntab, gacc)
+        e.addsofq(ntabq, sacc)
+    e.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.addgofr(ntab, gacc)
+        e.addsofq(ntabq, sacc)
+    e.sync()
+    dt = time.perf_counter() - t0
+    nw = e.numstack()
+    return dict(value=steps * nw / dt, ms_per_step=1e3 * dt / steps, population=nw,
+                nq=len(e.sofq_kvec(ntabq)[1]))
+
+
 def traffic_from_profiles(workload):
     """HBM bytes per launch of the dominant kernel from the rocprofv3 --pmc summaries in
     profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by its reading for a copy of known
@@ -312,7 +335,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed",
-                    choices=["mixed", "fp64", "allparticle", "batch"])
+                    choices=["mixed", "fp64", "allparticle", "batch", "observables"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy three times (counter calibration) and exit")
@@ -340,6 +363,17 @@ def main():
                                                      "mixed, single-particle moves, k=N",
                                              "dt": DT, "dcut": DCUT_SM},
             "batch": r}), flush=True)
+        return
+    if a.workload == "observables":
+        r = observables_run(a.steps, a.warmup)
+        print(json.dumps({
+            "metric": "walkers/sec through g(r) + density (100 bins) and S(q) (10 shells)",
+            "value": r["value"], "unit": "walkers/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic: reference initial walkers after 3 DMC steps",
+            "config": {"workload": f"observables N={N} {r['population']} walkers, "
+                                   f"ntabg=100, ntabq=10 ({r['nq']} q vectors)"}}), flush=True)
         return
     main_run = timed_run(a.workload, a.steps, a.warmup)
     print(f"bench: {a.workload}: {main_run}", file=sys.stderr, flush=True)

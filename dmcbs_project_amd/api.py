@@ -243,6 +243,36 @@ class Engine:
         L.check(self.lib.dmc_acceptance(self.h, C.byref(a), C.byref(p)))
         return a.value, p.value
 
+    # --- operators.f90: g(r), density profile, S(q) --------------------------------------
+    def addgofr(self, ntab, acc=None):
+        """addgofr (gofr.f90:52-98) of every walker of the resident population, added into
+        acc = (gnow, gerr [ntab+1][3], dnow, derr [2 ntab+1][6], wsblk [1]); returns acc."""
+        if acc is None:
+            acc = (np.zeros((ntab + 1, 3)), np.zeros((ntab + 1, 3)),
+                   np.zeros((2 * ntab + 1, 6)), np.zeros((2 * ntab + 1, 6)), np.zeros(1))
+        L.check(self.lib.dmc_gofr(self.h, int(ntab), *(a.reshape(-1) for a in acc)))
+        return acc
+
+    def sofq_kvec(self, ntabq):
+        """setupk as setupsofq calls it (sofq.f90:26): (numsh [ntabq], ak [nqtot][3])"""
+        nq = C.c_int32()
+        L.check(self.lib.dmc_sofq_kvec(self.h, int(ntabq), 0, C.byref(nq), None, None))
+        numsh = np.zeros(ntabq, np.int32)
+        ak = np.zeros((nq.value, 3))
+        L.check(self.lib.dmc_sofq_kvec(self.h, int(ntabq), nq.value, C.byref(nq),
+                                       numsh.ctypes.data, ak.ctypes.data))
+        return numsh, ak
+
+    def addsofq(self, ntabq, acc=None):
+        """addsofq (sofq.f90:40-65) of every walker of the resident population, added into
+        acc = (snow, serr [nqtot][3], wsblk [1]); returns acc."""
+        if acc is None:
+            nq = C.c_int32()
+            L.check(self.lib.dmc_sofq_kvec(self.h, int(ntabq), 0, C.byref(nq), None, None))
+            acc = (np.zeros((nq.value, 3)), np.zeros((nq.value, 3)), np.zeros(1))
+        L.check(self.lib.dmc_sofq(self.h, int(ntabq), *(a.reshape(-1) for a in acc)))
+        return acc
+
     def kernel_timing(self, enable=True):
         L.check(self.lib.dmc_kernel_timing(self.h, int(bool(enable))))
 

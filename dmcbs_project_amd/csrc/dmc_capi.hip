@@ -77,6 +77,10 @@ struct dmc_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int64_t acc_moves = 0, prop_moves = 0;
+    // q vectors of S(q) for the last shell count asked for (sofq_ntabq = 0: none yet)
+    int sofq_ntabq = 0;
+    std::vector<std::array<int, 3>> sofq_ik;
+    std::vector<int> sofq_numsh;
     bool pending = false;  // a propagation whose branch (factor 1) has not run yet
     int nev_pending = 0;   // timed move-kernel chunks of that propagation
 
@@ -232,6 +236,80 @@ int compact(dmc_ctx *c, const DevState &src, const DevState &dst, int n, int &to
                                           " exceeds capacity " + std::to_string(c->capacity));
     LAUNCHCHK(launch_map_gather(c->ds, src, dst, c->T, n, total, c->capacity, c->st));
     return 0;
+}
+
+// q vectors of S(q) for ntabq shells (cached per shell count)
+int sofq_vectors(dmc_ctx *c, int ntabq) {
+    if (ntabq < 1) return fail(DMC_ERR_ARG, "ntabq must be positive");
+    if (c->sofq_ntabq == ntabq) return DMC_OK;
+    std::string err;
+    c->sofq_ntabq = 0;
+    int st = enumerate_kshells(ntabq, c->sofq_ik, c->sofq_numsh, err);
+    if (st) return fail(st, err);
+    c->sofq_ntabq = ntabq;
+    return DMC_OK;
+}
+
+// Largest dynamic LDS of one observables workgroup.
+constexpr size_t kObsLdsMax = 64 * 1024;
+
+// g(r)/density (ntab > 0) or S(q) (ntabq > 0) sums of the resident population, in the
+// entry order of ObsArgs
+int run_obs(dmc_ctx *c, int ntab, int ntabq, std::vector<double> &sums) {
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
+    const int nw = c->nw, np = c->sys.npart, nup = c->sys.n;
+    int nq = 0, mmax = 0;
+    std::vector<int4> qm;
+    if (ntabq > 0) {
+        int st = sofq_vectors(c, ntabq);
+        if (st) return st;
+        nq = int(c->sofq_ik.size());
+        qm.resize(nq);
+        for (int q = 0; q < nq; ++q) {
+            const auto &m = c->sofq_ik[q];
+            qm[q] = make_int4(m[0], m[1], m[2], 0);
+            for (int d = 0; d < 3; ++d) mmax = std::max(mmax, std::abs(m[d]));
+        }
+    }
+    const int ng = ntab > 0 ? 3 * (ntab + 1) + 6 * (2 * ntab + 1) : 0;
+    const int nsum = 1 + ng + 6 * nq;
+    sums.assign(nsum, 0.0);
+    if (nw == 0) return DMC_OK;
+    if (obs_lds_bytes(np, ntab, nq, mmax) > kObsLdsMax)
+        return fail(DMC_ERR_ARG, "too many bins or q vectors for the observables kernel");
+    const int nblk = obs_blocks(nw), nseg = (nblk + 31) / 32;
+    const size_t nd = (size_t(nblk + nseg + 1) * nsum + 1) & ~size_t(1);
+    int e = c->ensure_scratch(sizeof(double) * nd + sizeof(int4) * size_t(nq) + 256);
+    if (e) return e;
+    double *part = static_cast<double *>(c->scratch), *tmp = part + size_t(nblk) * nsum,
+           *out = tmp + size_t(nseg) * nsum;
+    int4 *dq = reinterpret_cast<int4 *>(part + nd);
+    if (nq > 0)
+        HIPCHK(hipMemcpyAsync(dq, qm.data(), sizeof(int4) * nq, hipMemcpyHostToDevice, c->st));
+    ObsArgs a{};
+    a.x = c->A.x;
+    a.weight = c->A.weight;
+    a.wt0 = c->A.wt0;
+    a.is = c->A.is;
+    a.nw = nw;
+    a.npart = np;
+    a.nup = nup;
+    a.el = c->sys.el;
+    a.eli = c->sys.eli;
+    a.half = 0.5 * c->sys.el;
+    a.scale = ntab > 0 ? 1.0 / (0.5 * c->sys.el / ntab) : 0.0;  // gofr.f90:35-36
+    a.ntab = ntab;
+    a.qm = dq;
+    a.nq = nq;
+    a.mmax = mmax;
+    a.twopi_l = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;
+    a.nsum = nsum;
+    a.nblk = nblk;
+    a.part = part;
+    LAUNCHCHK(launch_obs(a, tmp, out, c->st));
+    HIPCHK(hipMemcpyAsync(sums.data(), out, sizeof(double) * nsum, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
 }
 
 }  // namespace
@@ -663,6 +741,61 @@ int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed) {
     HIPCHK(hipStreamSynchronize(c->st));
     if (accepted) *accepted = m[0];
     if (proposed) *proposed = m[1];
+    return DMC_OK;
+}
+
+int dmc_gofr(dmc_ctx *c, int32_t ntab, double *gnow, double *gerr, double *dnow, double *derr,
+             double *wsblk) {
+    if (!c || !gnow || !gerr || !dnow || !derr || !wsblk) return fail(DMC_ERR_ARG, "null argument");
+    if (ntab < 1) return fail(DMC_ERR_ARG, "ntab must be positive");
+    std::vector<double> sums;
+    int st = run_obs(c, ntab, 0, sums);
+    if (st) return st;
+    *wsblk += sums[0];
+    const double *g = sums.data() + 1, *d = g + 3 * (ntab + 1);
+    for (int e = 0; e < 3 * (ntab + 1); ++e) {
+        gnow[e] += 2.0 * g[e];
+        gerr[e] += 4.0 * g[e];
+    }
+    for (int b = 0; b <= 2 * ntab; ++b)
+        for (int ch = 0; ch < 6; ++ch) {
+            const int e = 6 * b + ch;
+            dnow[e] += (b == ntab ? 0.5 : 1.0) * d[e];
+            derr[e] += (b == ntab ? 0.25 : 1.0) * d[e];
+        }
+    return DMC_OK;
+}
+
+int dmc_sofq_kvec(dmc_ctx *c, int32_t ntabq, int32_t cap, int32_t *nqtot, int32_t *numsh,
+                  double *ak) {
+    if (!c || !nqtot) return fail(DMC_ERR_ARG, "null argument");
+    int st = sofq_vectors(c, ntabq);
+    if (st) return st;
+    const int nq = int(c->sofq_ik.size());
+    *nqtot = nq;
+    if (numsh)
+        for (int i = 0; i < ntabq; ++i) numsh[i] = c->sofq_numsh[i];
+    if (ak) {
+        if (cap < nq) return fail(DMC_ERR_ARG, "output capacity smaller than the q vectors");
+        const double piel = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;  // kshell.f90:137-138
+        for (int q = 0; q < nq; ++q)
+            for (int d = 0; d < 3; ++d) ak[3 * q + d] = piel * c->sofq_ik[q][d];
+    }
+    return DMC_OK;
+}
+
+int dmc_sofq(dmc_ctx *c, int32_t ntabq, double *snow, double *serr, double *wsblk) {
+    if (!c || !snow || !serr || !wsblk) return fail(DMC_ERR_ARG, "null argument");
+    std::vector<double> sums;
+    int st = run_obs(c, 0, ntabq, sums);
+    if (st) return st;
+    *wsblk += sums[0];
+    const int nq = int(c->sofq_ik.size());
+    for (int q = 0; q < nq; ++q)
+        for (int k = 0; k < 3; ++k) {
+            snow[3 * q + k] += sums[1 + 6 * q + k];
+            serr[3 * q + k] += sums[1 + 6 * q + 3 + k];
+        }
     return DMC_OK;
 }
 

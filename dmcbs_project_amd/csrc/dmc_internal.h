@@ -38,6 +38,10 @@ struct System {
     std::vector<double> vorb;
 };
 
+// The first nsh0 shells of half-space integer vectors in the order of kstates3d
+// (kshell.f90:20-156): ik (units of 2 pi / L), numsh[nsh0] vectors per shell.
+int enumerate_kshells(int nsh0, std::vector<std::array<int, 3>> &ik, std::vector<int> &numsh,
+                      std::string &err);
 // Builds everything from the parameters; returns a DMC_* status.
 int build_system(const dmc_params &p, System &s, std::string &err);
 // Rebuilds the beta table for new (b, d) (initbeta, betafun.f90:359-410).

@@ -54,6 +54,30 @@ int launch_pop_begin(const DevStep &T, hipStream_t st);
 
 bool nup_supported(int nup);
 
+// Observables of a walker population (addgofr, addsofq): per-block partial sums
+// part[nblk][nsum], entry 0 the summed weight, then (ntab > 0) the pair counts times weight
+// [ntab+1][3] and the density counts times weight [2 ntab+1][6], then (nq > 0)
+// [nq][6] = |rho_up|^2, |rho_dn|^2, Re rho_up conj(rho_dn) and their squares (real parts),
+// each times weight.
+struct ObsArgs {
+    const double *x, *weight, *wt0;
+    const int *is;
+    int nw, npart, nup;
+    double el, eli, half, scale;  // box side, 1/L, L/2, 1/dr with dr = L/(2 ntab)
+    int ntab;                // 0: no g(r) and density
+    const int4 *qm;          // S(q) vectors in units of 2 pi / L
+    int nq, mmax;            // 0: no S(q); largest |component| of the vectors
+    double twopi_l;
+    int nsum, nblk;
+    double *part;
+};
+constexpr int kObsBlocks = 1024;
+size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax);
+int obs_blocks(int nw);
+// k_obs into a.part, then a fixed-order reduction over the blocks: tmp holds
+// ceil(nblk/32) rows of nsum, out one row
+int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st);
+
 }  // namespace dmc
 
 namespace dmc {

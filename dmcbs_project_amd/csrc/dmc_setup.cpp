@@ -22,11 +22,14 @@ inline double lcg(uint64_t &s, uint64_t m, uint64_t a) {
 
 inline int isq(const std::array<int, 3> &v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
 
+}  // namespace
+
 // Half-space integer vectors grouped in shells of equal |k|^2, in the order of
 // kstates3d (kshell.f90:20-156): enumeration over the positive octant with the sign
 // variants that complete a half space, then an exchange sort on |k|^2.
-int kshells(System &s, std::string &err) {
-    const int nsh0 = s.p.norb;
+
+int enumerate_kshells(int nsh0, std::vector<std::array<int, 3>> &ik, std::vector<int> &numsh,
+                      std::string &err) {
     const int i2max = 2 * nsh0;
     const int imax = int(std::lround(std::sqrt(float(i2max))));
     std::vector<std::array<int, 3>> v;
@@ -55,22 +58,29 @@ int kshells(System &s, std::string &err) {
         ++count[nsh];
     }
     if (nsh0 >= nsh) {
-        err = "Too many states: norb exceeds the enumerated shells";
+        err = "Too many states: the shell count exceeds the enumerated shells";
         return DMC_ERR_SETUP;
     }
-    s.numsh.assign(nsh0, 0);
-    s.ik.clear();
-    s.kshell.clear();
+    numsh.assign(nsh0, 0);
+    ik.clear();
     int k = 0;
     for (int sh = 0; sh < nsh0; ++sh) {
-        s.numsh[sh] = count[sh + 1];
+        numsh[sh] = count[sh + 1];
         const int m = std::max(1, count[sh + 1]);
-        for (int j = 0; j < m; ++j, ++k) {
-            s.ik.push_back(v[k]);
-            s.kshell.push_back(sh);
-        }
+        for (int j = 0; j < m; ++j, ++k) ik.push_back(v[k]);
     }
-    s.nk = k;
+    return DMC_OK;
+}
+
+namespace {
+
+int kshells(System &s, std::string &err) {
+    int st = enumerate_kshells(s.p.norb, s.ik, s.numsh, err);
+    if (st) return st;
+    s.kshell.clear();
+    for (int sh = 0; sh < s.p.norb; ++sh)
+        for (int j = 0; j < std::max(1, s.numsh[sh]); ++j) s.kshell.push_back(sh);
+    s.nk = int(s.ik.size());
     if (s.nk - 1 > kMaxQ) {
         err = "too many plane waves for the device kernels";
         return DMC_ERR_ARG;

@@ -139,6 +139,36 @@ int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double
 int dmc_opt_accumulate(dmc_ctx *c);
 int dmc_opt_sums(dmc_ctx *c, double *sums, int32_t reset);
 
+/* Observables of the resident population (the operators of operators.f90:92-108, called
+ * once per block after the steps, dmcbcs.f90:465-471).  Each walker carries the weight of
+ * addoperators (operators.f90:102): weight * is * wt0 (psil = psigl on this path).  The sums
+ * are ADDED into the caller's accumulators, as addgofr/addsofq add into the module's; the
+ * block normalisation (updategofr gofr.f90:100-134, updatesofq sofq.f90:67-89) and the
+ * output tables (writegofr gofr.f90:136-177, writesofq sofq.f90:91-119) are host arithmetic.
+ * Arrays follow the reference's column-major layout:
+ *
+ * addgofr (gofr.f90:52-98) with ndim = 3 and ntab bins of width dr = L/(2 ntab):
+ *   gnow, gerr [ntab+1][3]    gnow(itab, index) with itab 1 up-down, 2 up-up, 3 down-down;
+ *                             index ntab collects the pairs at r >= L/2
+ *   dnow, derr [2 ntab+1][6]  dnow(itab, index), index = -ntab..ntab (row index + ntab);
+ *                             itab 1-3 the x, y, z profiles of spin up, 4-6 of spin down
+ *   wsblk                     the summed weight
+ * The kernel accumulates one weighted count per bin; gnow, gerr, dnow and derr follow from it
+ * with the reference's factors (2 and 4 per pair; 1, or 0.5 and 0.25 in the central
+ * density bin). */
+int dmc_gofr(dmc_ctx *c, int32_t ntab, double *gnow, double *gerr, double *dnow, double *derr,
+             double *wsblk);
+/* setupk with ndim = 3 as setupsofq calls it (sofq.f90:26, kshell.f90:6-156): the q vectors
+ * of S(q) for ntabq shells of this context's box.  numsh[ntabq] vectors per shell,
+ * ak[nqtot][3] = 2 pi m / L with at most cap rows, *nqtot = the number of vectors. */
+int dmc_sofq_kvec(dmc_ctx *c, int32_t ntabq, int32_t cap, int32_t *nqtot, int32_t *numsh,
+                  double *ak);
+/* addsofq (sofq.f90:40-65) for the q vectors of dmc_sofq_kvec(ntabq):
+ *   snow, serr [nqtot][3]  (|rho_up|^2, |rho_dn|^2, Re rho_up conj(rho_dn)) * weight and
+ *                          the real parts of their squares * weight
+ *   wsblk                  the summed weight */
+int dmc_sofq(dmc_ctx *c, int32_t ntabq, double *snow, double *serr, double *wsblk);
+
 /* Timing of the dominant kernel with HIP events recorded on the stream it is launched on:
  * the move kernel (k_sm_moves, every chunk of krefresh moves) for single-particle moves, the
  * fused step kernel (k_step) for all-particle moves.  enable = 1 starts collecting, 0 stops.
