@@ -128,6 +128,27 @@ module dmc_gpu
          real(c_double), intent(out) :: sums(*)
          integer(c_int32_t), value :: reset
       end function
+      integer(c_int) function dmc_gofr(ctx, ntab, gnow, gerr, dnow, derr, wsblk) &
+            bind(c, name='dmc_gofr')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ntab
+         real(c_double), intent(inout) :: gnow(*), gerr(*), dnow(*), derr(*), wsblk
+      end function
+      integer(c_int) function dmc_sofq_kvec(ctx, ntabq, cap, nqtot, numsh, ak) &
+            bind(c, name='dmc_sofq_kvec')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ntabq, cap
+         integer(c_int32_t), intent(out) :: nqtot
+         type(c_ptr), value :: numsh, ak
+      end function
+      integer(c_int) function dmc_sofq(ctx, ntabq, snow, serr, wsblk) bind(c, name='dmc_sofq')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ntabq
+         real(c_double), intent(inout) :: snow(*), serr(*), wsblk
+      end function
       integer(c_int) function dmc_sync(ctx) bind(c, name='dmc_sync')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

@@ -16,6 +16,7 @@ program dmcbcs_gpu
    use step
    use estimator
    use optimizer
+   use operators
    implicit none
    integer(kind=i8) :: irn,irn0
    integer(kind=i4) :: npart,nwalk,idmc,neq,nav,nstep,ndim,ntab,nbin(2)
@@ -24,7 +25,7 @@ program dmcbcs_gpu
    logical :: vmc
    real(kind=r8) :: hbar,dt,etrial,etrial0,popup,popdn,dcut,dobal,wtcut,mass(2)
    real(kind=r8) :: el,mu,eunit,kunit,rho,pi,tau,tau0
-   logical :: isite,units,fn,euclid,empty
+   logical :: isite,units,fn,euclid,empty,operat
    character(len=90), allocatable :: answer(:)
    character(len=32) :: arg
    real(kind=r8) :: timebl0,timebl
@@ -92,16 +93,18 @@ program dmcbcs_gpu
       eunit=3.0_r8/(8.0_r8-ndim)*hbar/(2.0_r8*mu)*kunit**2
    else
       eunit=1.0_r8
+      kunit=1.0_r8
    endif
    etrial=etrial*eunit
    if (el.ne.0.0_r8) etrial=etrial*npart
-! the four operator table sizes (operators.f90:33-41); observables are not computed
-   do i=1,4
-      read (5,*) idum
-   enddo
+! the four operator table sizes (operators.f90:33-41): g(r) and S(q) on the device
+   call setoperators(ndim,nbin,el,operat,kunit)
    euclid=.false.
    optiter=1
-   if (idmc.eq.-1) call setupopt(npart,optiter)
+   if (idmc.eq.-1) then
+      call setupopt(npart,optiter)
+      operat=.false.  ! no operators during the optimisation (dmcbcs.f90:173-176)
+   endif
    call setstep(idmc,ndim,npart,nbin,hbar,dt,etrial,el,mass,wtcut,dcut,eunit,fn)
    if (vmc) then
 ! the variational estimators of dmcbcs.f90:180-186
@@ -155,6 +158,12 @@ program dmcbcs_gpu
    endif
    write (6,'(''Configurations at tau ='',t35,f15.10)') tau0
    tau=tau0
+! the operators of the initial walkers (dmcbcs.f90:379-395)
+   if (operat.and.tau.eq.0.0_r8.and.idmc.ne.-1.and.idmc.ne.0) then
+      call addoperators
+      call updateoperators
+      call writeoperators(tau)
+   endif
 
 ! optimisation iterations (dmcbcs.f90:234-539): after the first, the walkers' stored
 ! wavefunction values are recomputed with the new parameters (dmcbcs.f90:314-353)
@@ -169,6 +178,7 @@ program dmcbcs_gpu
       if (i.eq.neq+1) then
          if (neq.gt.0) write (6,'(/,''equilibration done'')')
          call zerest
+         if (operat) call zerooperators
          it=0
       endif
       it=it+1
@@ -188,6 +198,13 @@ program dmcbcs_gpu
       answer=resstring(tau)
       write (6,'(a90)') (answer(k),k=1,size(answer))
       write (6,'(''Time for block ='',f10.3,'' seconds'')') timebl-timebl0
+! operators of the population at the end of each block past equilibration
+! (dmcbcs.f90:465-478)
+      if (operat.and.i.gt.neq) then
+         call addoperators
+         call updateoperators
+         call writeoperators(tau)
+      endif
 ! dmcbcs.f90:492-503: calder over the walkers after every block past equilibration
       if (idmc.eq.-1.and.i.gt.neq) call calder
    enddo

@@ -8,6 +8,9 @@
 !   step         setstep, step1, branch, writewtisto         (step.f90:11-334)
 !   estimator    setestnum, addest, zerest, update, resstring (estimator.f90:19-149)
 !   optimizer    setupopt, calder, updatepar                  (optimizer.f90:14-169)
+!   gofr, sofq   setup, zero, add, update, write              (gofr.f90:14-177, sofq.f90:14-119)
+!   operators    setoperators, zero/add/update/writeoperators (operators.f90:7-140, g(r)
+!                and S(q) only)
 ! All computation runs on the device through the C-ABI (module dmc_gpu); the walker
 ! population stays in device memory.  The deck values each reference module reads from
 ! unit 5 are read here in the same order and collected into one dmc_params record, and the
@@ -501,3 +504,251 @@ contains
    enddo
    end subroutine solve
 end module optimizer
+
+! g(r) and the density profile (gofr.f90:14-177).  The reference adds one walker per call
+! (addgofr); here addgofr_population adds every walker of the device population with the
+! weight of addoperators in one call of dmc_gofr.  Block normalisation and the tables are the
+! reference's host arithmetic on one process (its addall is the identity there).
+module gofr
+   use engine
+   implicit none
+   private
+   public :: setupgofr, zerogofr, addgofr_population, updategofr, writegofr
+   integer(kind=i4), save :: npart, ntab, ndim
+   real(kind=r8), save :: el, dr, wsblk, totalw, kf
+   real(kind=r8), save, allocatable :: gnow(:,:), gerr(:,:), g(:,:), ge(:,:)
+   real(kind=r8), save, allocatable :: dnow(:,:), derr(:,:), d(:,:), de(:,:)
+contains
+   subroutine setupgofr(ndin,ntabin,nbinin,elin,kfin)
+   integer(kind=i4) :: ndin,ntabin,nbinin(:)
+   real(kind=r8) :: elin,kfin
+   if (ndin.ne.3) error stop 'setupgofr: three dimensions only'
+   ndim=ndin
+   ntab=ntabin
+   npart=sum(nbinin)
+   el=elin
+   kf=kfin
+   allocate(gnow(3,0:ntab),gerr(3,0:ntab),g(3,0:ntab),ge(3,0:ntab))
+   allocate(dnow(6,-ntab:ntab),derr(6,-ntab:ntab),d(6,-ntab:ntab),de(6,-ntab:ntab))
+   dr=0.5_r8*el/ntab
+   call zerogofr
+   end subroutine setupgofr
+
+   subroutine zerogofr
+   g=0.0_r8
+   gnow=0.0_r8
+   gerr=0.0_r8
+   ge=0.0_r8
+   d=0.0_r8
+   dnow=0.0_r8
+   derr=0.0_r8
+   de=0.0_r8
+   wsblk=0.0_r8
+   end subroutine zerogofr
+
+   subroutine addgofr_population
+   call dmc_check(dmc_gofr(engine_ctx(),ntab,gnow,gerr,dnow,derr,wsblk),'dmc_gofr')
+   end subroutine addgofr_population
+
+   subroutine updategofr
+   real(kind=r8) :: norm
+   totalw=wsblk
+   norm=1.0_r8/wsblk
+   g=gnow*norm
+   ge=sqrt(abs(gerr*norm-g**2)/wsblk)
+   d=dnow*norm
+   de=sqrt(abs(derr*norm-d**2)/wsblk)
+   wsblk=0.0_r8
+   gnow=0.0_r8
+   gerr=0.0_r8
+   dnow=0.0_r8
+   derr=0.0_r8
+   end subroutine updategofr
+
+   subroutine writegofr(filename,tau)
+   character(len=*) :: filename
+   real(kind=r8) :: tau,pi,r0,r1,vol,rho,facl
+   integer(kind=i4) :: i
+   pi=4.0_r8*atan(1.0_r8)
+   rho=npart/el**ndim
+   open (unit=77,file=filename,position='append')
+   write (77,'(''# r*kF, g(r), err'')')
+   write (77,'(''# total weight = '',e15.7)') totalw
+   write (77,'(''# tau = '',e15.7)') tau
+   do i=0,ntab-1
+      r0=i*dr
+      r1=(i+1)*dr
+      vol=4.0_r8*pi*(r1**3-r0**3)/3.0_r8
+      facl=1.0_r8/(rho*npart*vol)
+      write (77,'(7e15.7)') (i+0.5_r8)*dr,g(1,i)*facl,ge(1,i)*facl,g(2,i)*facl,ge(2,i)*facl, &
+         g(3,i)*facl,ge(3,i)*facl
+   enddo
+   write (77,*) ''
+   write (77,*) ''
+   close (77)
+   open (unit=77,file='rho.out',position='append')
+   write (77,'(''# r, xrhoup, err, xrhodn, err, yrhoup, err, yrhodn, err, zrhoup, err, zrhodn, err'')')
+   write (77,'(''# total weight = '',e15.7)') totalw
+   write (77,'(''# tau = '',e15.7)') tau
+   facl=1.0_r8/npart
+   do i=-ntab,ntab
+      write (77,'(13e15.7)') i*dr,d(1,i)*facl,de(1,i)*facl,d(4,i)*facl,de(4,i)*facl, &
+         d(2,i)*facl,de(2,i)*facl,d(5,i)*facl,de(5,i)*facl,d(3,i)*facl,de(3,i)*facl, &
+         d(6,i)*facl,de(6,i)*facl
+   enddo
+   write (77,*) ''
+   write (77,*) ''
+   close (77)
+   end subroutine writegofr
+end module gofr
+
+! S(q) (sofq.f90:14-119): the q vectors of setupk come from the engine (dmc_sofq_kvec), the
+! sums of every walker from one call of dmc_sofq.  The deck is read before the engine's
+! context exists (setstep comes later), so the q vectors and the accumulators are set up at
+! the first zero or add.
+module sofq
+   use engine
+   implicit none
+   private
+   public :: setupsofq, zerosofq, addsofq_population, updatesofq, writesofq
+   integer(kind=i4), save :: npart, ntabq, nqtot
+   logical, save :: ready=.false.
+   integer(c_int32_t), save, allocatable, target :: numsh(:)
+   real(kind=r8), save :: wsblk, totalw, kf
+   real(c_double), save, allocatable, target :: ak(:,:)
+   real(kind=r8), save, allocatable :: snow(:,:), s(:,:), serr(:,:), se(:,:)
+contains
+   subroutine setupsofq(ndin,nbinin,el,ntabqin,kfin)
+   integer(kind=i4) :: ndin,nbinin(:),ntabqin
+   real(kind=r8) :: el,kfin
+   if (ndin.ne.3) error stop 'setupsofq: three dimensions only'
+   npart=sum(nbinin)
+   ntabq=ntabqin
+   kf=kfin
+   end subroutine setupsofq
+
+! setupk (kshell.f90:6-18) through the engine, and the accumulators
+   subroutine kvectors
+   integer(c_int32_t) :: nq
+   if (ready) return
+   allocate(numsh(ntabq))
+   call dmc_check(dmc_sofq_kvec(engine_ctx(),ntabq,0,nq,c_null_ptr,c_null_ptr),'dmc_sofq_kvec')
+   nqtot=nq
+   allocate(ak(3,nqtot))
+   call dmc_check(dmc_sofq_kvec(engine_ctx(),ntabq,nq,nq,c_loc(numsh),c_loc(ak)), &
+      'dmc_sofq_kvec')
+   allocate(snow(3,nqtot),s(3,nqtot),serr(3,nqtot),se(3,nqtot))
+   s=0.0_r8
+   snow=0.0_r8
+   serr=0.0_r8
+   se=0.0_r8
+   wsblk=0.0_r8
+   ready=.true.
+   end subroutine kvectors
+
+   subroutine zerosofq
+   call kvectors
+   s=0.0_r8
+   snow=0.0_r8
+   serr=0.0_r8
+   se=0.0_r8
+   wsblk=0.0_r8
+   end subroutine zerosofq
+
+   subroutine addsofq_population
+   call kvectors
+   call dmc_check(dmc_sofq(engine_ctx(),ntabq,snow,serr,wsblk),'dmc_sofq')
+   end subroutine addsofq_population
+
+   subroutine updatesofq
+   real(kind=r8) :: norm
+   totalw=wsblk
+   norm=1.0_r8/wsblk
+   s=snow*norm
+   se=sqrt(abs(serr*norm-s**2)/wsblk)
+   wsblk=0.0_r8
+   snow=0.0_r8
+   serr=0.0_r8
+   end subroutine updatesofq
+
+   subroutine writesofq(filename,tau)
+   character(len=*) :: filename
+   real(kind=r8) :: tau,ave(3),err(3),q
+   integer(kind=i4) :: iq,j,k
+   open (unit=79,file=filename,position='append')
+   write (79,'(''# q/kF, s(q), err'')')
+   write (79,'(''# total weight = '',e15.7)') totalw
+   write (79,'(''# tau = '',e15.7)') tau
+   k=0
+   do iq=1,ntabq
+      ave=0.0_r8
+      err=0.0_r8
+      do j=1,numsh(iq)
+         k=k+1
+         ave=ave+s(:,k)
+         err=err+se(:,k)
+      enddo
+      ave=ave/numsh(iq)/npart
+      err=err/numsh(iq)/npart
+      q=sqrt(sum(ak(:,k)**2))
+      write (79,'(7e15.7)') q/kf,ave(1),err(1),ave(2),err(2),ave(3),err(3)
+   enddo
+   write (79,*) ' '
+   write (79,*) ' '
+   close (79)
+   end subroutine writesofq
+end module sofq
+
+! operators.f90:7-140 for a periodic box: the four table sizes of the deck; g(r) and S(q)
+! are computed, n(k) and the euclidean density response stop the run.  The walker loops of
+! dmcbcs.f90:385-390,465-471 (pop, addoperators, push) become one call per observable over
+! the device population.
+module operators
+   use engine
+   use gofr
+   use sofq
+   implicit none
+   private
+   public :: setoperators, zerooperators, addoperators, updateoperators, writeoperators
+   logical, save :: igofr=.false., isofq=.false.
+contains
+   subroutine setoperators(ndim,nbin,el,oper,kunit)
+   integer(kind=i4) :: ndim,nbin(:)
+   real(kind=r8) :: el,kunit
+   logical :: oper
+   integer(kind=i4) :: ntabg,ntabq,ntabenk,ntabqeuc
+   if (el.le.0.0_r8) error stop 'setoperators: periodic boxes only'
+   read (5,*) ntabg
+   read (5,*) ntabq
+   read (5,*) ntabenk
+   read (5,*) ntabqeuc
+   if (ntabenk.gt.0.or.ntabqeuc.gt.0) &
+      error stop 'setoperators: n(k) and the euclidean density response are not supported'
+   igofr=ntabg.gt.0
+   isofq=ntabq.gt.0
+   if (igofr) call setupgofr(ndim,ntabg,nbin,el,kunit)
+   if (isofq) call setupsofq(ndim,nbin,el,ntabq,kunit)
+   oper=igofr.or.isofq
+   end subroutine setoperators
+
+   subroutine zerooperators
+   if (igofr) call zerogofr
+   if (isofq) call zerosofq
+   end subroutine zerooperators
+
+   subroutine addoperators
+   if (igofr) call addgofr_population
+   if (isofq) call addsofq_population
+   end subroutine addoperators
+
+   subroutine updateoperators
+   if (igofr) call updategofr
+   if (isofq) call updatesofq
+   end subroutine updateoperators
+
+   subroutine writeoperators(tau)
+   real(kind=r8) :: tau
+   if (igofr) call writegofr('gofr.out',tau)
+   if (isofq) call writesofq('sofq.out',tau)
+   end subroutine writeoperators
+end module operators

@@ -85,3 +85,53 @@ def test_driver_optimisation_run_matches_reference(tmp_path):
     np.testing.assert_allclose(par, g["opt_par"], rtol=1e-7, atol=1e-12)
     ene = float(next(l for l in lines if "eave" in l).split()[2])
     assert ene == pytest.approx(float(g["opt_ene"]), rel=1e-10)
+
+
+def _tables(path, header):
+    """numeric tables of an appended observables file, one per header line"""
+    tabs, cur = [], None
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if t.startswith(header):
+                cur = []
+                tabs.append(cur)
+            elif t and not t.startswith("#") and cur is not None:
+                cur.append([float(v) for v in t.split()])
+    return [np.array(t) for t in tabs]
+
+
+def _close_printed(a, b):
+    scale = np.maximum(np.abs(b), 1e-6 * np.abs(b).max(axis=0, keepdims=True))
+    err = np.abs(a - b) / np.where(scale > 0, scale, 1.0)
+    assert err.max() < 1e-6, err.max()
+
+
+def test_driver_observables(tmp_path, oracle_mod):
+    """g(r), density profile and S(q) through the driver's operators module (dmc_gofr,
+    dmc_sofq): the tau = 0 tables of the initial walkers (dmcbcs.f90:379-395) equal the
+    oracle's addgofr/addsofq of the same walkers with the reference's normalisation, and each
+    block past equilibration appends one table (dmcbcs.f90:465-478)."""
+    from oracle.refdeck import make_deck
+    from dmcbs_project_amd import observables as obs
+    n, nwalk, ntg, ntq, seed = 14, 300, 25, 5, 12345
+    deck, _ = make_deck(npart=n, norb=3, nwalk=nwalk, neq=1, nav=2, nstep=5, ntabg=ntg,
+                        ntabq=ntq, seed=seed)
+    run(deck, tmp_path)
+    gt = _tables(tmp_path / "gofr.out", "# r*kF")
+    rt = _tables(tmp_path / "rho.out", "# r, xrhoup")
+    st = _tables(tmp_path / "sofq.out", "# q/kF")
+    assert len(gt) == len(rt) == len(st) == 3
+    s = oracle_mod.System(npart=n, norb=3)
+    x, _ = s.init_walkers(seed, nwalk)
+    w = np.ones(nwalk)
+    g, ge, d, de, ws = obs.updategofr(*oracle_mod.addgofr(x, w, s.el, ntg))
+    _close_printed(gt[0], obs.gofr_rows(g, ge, n, s.el, ntg))
+    _close_printed(rt[0], obs.rho_rows(d, de, n, s.el, ntg))
+    numsh, ak = oracle_mod.kshells(s.el, ntq)
+    sq, se, _ = obs.updatesofq(*oracle_mod.addsofq(x, w, ak))
+    _close_printed(st[0], obs.sofq_rows(sq, se, numsh, ak, n, obs.kfermi(n, s.el)))
+    for t in gt[1:]:
+        assert t.shape == (ntg, 7) and np.all(np.isfinite(t))
+    for t in st[1:]:
+        np.testing.assert_allclose(t[0, 1], (n // 2) ** 2 / n, rtol=1e-6)  # S(0) = n_up^2/N
