@@ -273,6 +273,24 @@ class Engine:
         L.check(self.lib.dmc_sofq(self.h, int(ntabq), *(a.reshape(-1) for a in acc)))
         return acc
 
+    def nofk_shells(self, ntab):
+        """setupenk (momentum3d.f90:13-79): (k of each shell [nsh], vectors per shell [nsh])"""
+        n = C.c_int32()
+        L.check(self.lib.dmc_nofk_shells(self.h, int(ntab), 0, C.byref(n), None, None))
+        kmag, nk = np.zeros(n.value), np.zeros(n.value, np.int32)
+        L.check(self.lib.dmc_nofk_shells(self.h, int(ntab), n.value, C.byref(n),
+                                         kmag.ctypes.data, nk.ctypes.data))
+        return kmag, nk
+
+    def addenk(self, ntab, acc=None):
+        """addenk (momentum3d.f90:89-120) of every walker of the resident population, added
+        into acc = (enknow [nsh], enkerr [nsh], wsblk [1]); returns acc."""
+        if acc is None:
+            n = len(self.nofk_shells(ntab)[1])
+            acc = (np.zeros(n), np.zeros(n), np.zeros(1))
+        L.check(self.lib.dmc_nofk(self.h, int(ntab), *acc))
+        return acc
+
     def kernel_timing(self, enable=True):
         L.check(self.lib.dmc_kernel_timing(self.h, int(bool(enable))))
 

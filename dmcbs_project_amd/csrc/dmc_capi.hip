@@ -252,6 +252,8 @@ int sofq_vectors(dmc_ctx *c, int ntabq) {
 
 // Largest dynamic LDS of one observables workgroup.
 constexpr size_t kObsLdsMax = 64 * 1024;
+// n(k) shells up to |m|^2 = 63: every component <= 7, the kernel's cosine table
+constexpr int kNofkMaxTab = 63;
 
 // g(r)/density (ntab > 0) or S(q) (ntabq > 0) sums of the resident population, in the
 // entry order of ObsArgs
@@ -796,6 +798,97 @@ int dmc_sofq(dmc_ctx *c, int32_t ntabq, double *snow, double *serr, double *wsbl
             snow[3 * q + k] += sums[1 + 6 * q + k];
             serr[3 * q + k] += sums[1 + 6 * q + 3 + k];
         }
+    return DMC_OK;
+}
+
+int dmc_nofk_shells(dmc_ctx *c, int32_t ntab, int32_t cap, int32_t *nsh, double *kmag,
+                    int32_t *nk) {
+    if (!c || !nsh) return fail(DMC_ERR_ARG, "null argument");
+    if (ntab < 0 || ntab > kNofkMaxTab) return fail(DMC_ERR_ARG, "ntab of n(k) must be 0..63");
+    std::vector<std::array<int, 3>> ik;
+    std::vector<int> first, count;
+    enk_shells(ntab, ik, first, count);
+    const int n = int(count.size());
+    *nsh = n;
+    if ((kmag || nk) && cap < n) return fail(DMC_ERR_ARG, "output capacity smaller than the shells");
+    const double dk = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;
+    for (int k = 0; k < n; ++k) {
+        const auto &m = ik[first[k]];
+        if (kmag) kmag[k] = std::sqrt(dk * dk * double(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+        if (nk) nk[k] = count[k];
+    }
+    return DMC_OK;
+}
+
+int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *wsblk) {
+    if (!c || !enknow || !enkerr || !wsblk) return fail(DMC_ERR_ARG, "null argument");
+    if (ntab < 0 || ntab > kNofkMaxTab) return fail(DMC_ERR_ARG, "ntab of n(k) must be 0..63");
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
+    std::vector<std::array<int, 3>> ik;
+    std::vector<int> first, count;
+    enk_shells(ntab, ik, first, count);
+    const int nsh = int(count.size()), ntot = int(ik.size()), nw = c->nw, np = c->sys.npart;
+    const int nsamp = 10;  // momentum3d.f90:22
+    const int ncol = 1 + 2 * nsh, nmove = np * nsamp;
+    if (nw == 0) return DMC_OK;
+    if (nofk_lds_bytes(c->sys.n, c->ds.nq, nsh) > kObsLdsMax)
+        return fail(DMC_ERR_ARG, "too many n(k) shells for the kernel's LDS");
+    // state after 3 m draws of ran1, m < npart nsamp
+    std::vector<uint64_t> ja(nmove), jc(nmove);
+    {
+        std::vector<uint64_t> a1, c1;
+        ran1_jump_table(3 * nmove, a1, c1);
+        ja[0] = 1;
+        jc[0] = 0;
+        for (int m = 1; m < nmove; ++m) {
+            ja[m] = a1[3 * m - 1];
+            jc[m] = c1[3 * m - 1];
+        }
+    }
+    std::vector<int4> hik(ntot);
+    for (int v = 0; v < ntot; ++v) hik[v] = make_int4(ik[v][0], ik[v][1], ik[v][2], 0);
+    std::vector<int2> hsh(nsh);
+    for (int k = 0; k < nsh; ++k) hsh[k] = make_int2(first[k], count[k]);
+    const int seg = 256, nseg = (nw + seg - 1) / seg;
+    const size_t nd = (size_t(nw + nseg + 1) * ncol + 1) & ~size_t(1);
+    const size_t bytes = sizeof(double) * nd + 2 * sizeof(uint64_t) * nmove +
+                         sizeof(int4) * ntot + sizeof(int2) * nsh + 256;
+    int e = c->ensure_scratch(bytes);
+    if (e) return e;
+    double *rows = static_cast<double *>(c->scratch), *tmp = rows + size_t(nw) * ncol,
+           *out = tmp + size_t(nseg) * ncol;
+    uint64_t *dja = reinterpret_cast<uint64_t *>(rows + nd), *djc = dja + nmove;
+    int4 *dik = reinterpret_cast<int4 *>(djc + nmove);
+    int2 *dsh = reinterpret_cast<int2 *>(dik + ntot);
+    HIPCHK(hipMemcpyAsync(dja, ja.data(), 8 * nmove, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(djc, jc.data(), 8 * nmove, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dik, hik.data(), sizeof(int4) * ntot, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dsh, hsh.data(), sizeof(int2) * nsh, hipMemcpyHostToDevice, c->st));
+    NofkArgs a{};
+    a.x = c->A.x;
+    a.weight = c->A.weight;
+    a.wt0 = c->A.wt0;
+    a.is = c->A.is;
+    a.irn = c->A.irn;
+    a.nw = nw;
+    a.nsamp = nsamp;
+    a.nsh = nsh;
+    a.ik = dik;
+    a.shell = dsh;
+    a.ja = dja;
+    a.jc = djc;
+    a.dk = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;
+    a.rows = rows;
+    LAUNCHCHK(launch_nofk(c->sys.n, c->ds, a, c->st));
+    LAUNCHCHK(launch_rows_reduce(rows, nw, ncol, seg, tmp, out, c->st));
+    std::vector<double> sums(ncol);
+    HIPCHK(hipMemcpyAsync(sums.data(), out, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *wsblk += sums[0];
+    for (int k = 0; k < nsh; ++k) {
+        enknow[k] += sums[1 + k];
+        enkerr[k] += sums[1 + nsh + k];
+    }
     return DMC_OK;
 }
 

@@ -42,6 +42,11 @@ struct System {
 // (kshell.f90:20-156): ik (units of 2 pi / L), numsh[nsh0] vectors per shell.
 int enumerate_kshells(int nsh0, std::vector<std::array<int, 3>> &ik, std::vector<int> &numsh,
                       std::string &err);
+// The octant vectors of n(k) (setupenk, momentum3d.f90:13-79): ix, iy, iz >= 0 with
+// |m|^2 <= ntab in the reference's order, grouped in shells of equal |m|^2 (first index and
+// count of each shell).
+void enk_shells(int ntab, std::vector<std::array<int, 3>> &ik, std::vector<int> &first,
+                std::vector<int> &count);
 // Builds everything from the parameters; returns a DMC_* status.
 int build_system(const dmc_params &p, System &s, std::string &err);
 // Rebuilds the beta table for new (b, d) (initbeta, betafun.f90:359-410).

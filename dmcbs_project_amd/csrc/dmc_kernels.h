@@ -77,6 +77,26 @@ int obs_blocks(int nw);
 // k_obs into a.part, then a fixed-order reduction over the blocks: tmp holds
 // ceil(nblk/32) rows of nsum, out one row
 int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st);
+// out[e] = sum over the rows r of in[r][e] in row order, in two passes: segments of seg
+// rows into tmp [ceil(nrow/seg)][ncol], then the segments
+int launch_rows_reduce(const double *in, int nrow, int ncol, int seg, double *tmp, double *out,
+                       hipStream_t st);
+
+// n(k) of a walker population (addenk): one row per walker [nw][1 + 2 nsh] =
+// (wt nsamp, enktmp/nk wt [nsh], (enktmp/nk)^2 wt [nsh])
+struct NofkArgs {
+    const double *x, *weight, *wt0;
+    const int *is;
+    const uint64_t *irn;
+    int nw, nsamp, nsh;
+    const int4 *ik;          // octant vectors in shell order
+    const int2 *shell;       // (first vector, count) of each shell
+    const uint64_t *ja, *jc; // ran1 state after 3 m draws = ja[m] s + jc[m], m < npart nsamp
+    double dk;               // 2 pi / L
+    double *rows;
+};
+size_t nofk_lds_bytes(int nup, int nq, int nsh);
+int launch_nofk(int nup, const DevSys &S, const NofkArgs &a, hipStream_t st);
 
 }  // namespace dmc
 

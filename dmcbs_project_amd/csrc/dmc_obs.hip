@@ -194,11 +194,16 @@ int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st) {
     const size_t lds = obs_lds_bytes(a.npart, a.ntab, a.nq, a.mmax);
     hipLaunchKernelGGL(k_obs, dim3(a.nblk), dim3(kObsThreads), lds, st, a);
     // block partials [nblk][nsum] -> segments of 32 rows -> one row
-    const int seg = 32, nseg = (a.nblk + seg - 1) / seg;
-    const int gx = (a.nsum + kObsThreads - 1) / kObsThreads;
-    hipLaunchKernelGGL(k_obs_reduce, dim3(gx, nseg), dim3(kObsThreads), 0, st, a.part, a.nblk,
-                       a.nsum, seg, tmp);
-    hipLaunchKernelGGL(k_obs_reduce, dim3(gx, 1), dim3(kObsThreads), 0, st, tmp, nseg, a.nsum,
+    return launch_rows_reduce(a.part, a.nblk, a.nsum, 32, tmp, out, st);
+}
+
+int launch_rows_reduce(const double *in, int nrow, int ncol, int seg, double *tmp, double *out,
+                       hipStream_t st) {
+    const int nseg = (nrow + seg - 1) / seg;
+    const int gx = (ncol + kObsThreads - 1) / kObsThreads;
+    hipLaunchKernelGGL(k_obs_reduce, dim3(gx, nseg), dim3(kObsThreads), 0, st, in, nrow, ncol,
+                       seg, tmp);
+    hipLaunchKernelGGL(k_obs_reduce, dim3(gx, 1), dim3(kObsThreads), 0, st, tmp, nseg, ncol,
                        nseg, out);
     return int(hipGetLastError());
 }

@@ -72,6 +72,32 @@ int enumerate_kshells(int nsh0, std::vector<std::array<int, 3>> &ik, std::vector
     return DMC_OK;
 }
 
+// setupenk (momentum3d.f90:13-79): enumeration over the octant up to int(sqrt(ntab)) + 1 per
+// axis, then the reference's exchange pass (each later vector against every earlier one)
+// into |m|^2 order, then shells of equal |m|^2
+void enk_shells(int ntab, std::vector<std::array<int, 3>> &ik, std::vector<int> &first,
+                std::vector<int> &count) {
+    const int nnk = int(std::sqrt(double(ntab))) + 1;
+    ik.clear();
+    for (int ix = 0; ix <= nnk; ++ix)
+        for (int iy = 0; iy <= nnk; ++iy)
+            for (int iz = 0; iz <= nnk; ++iz)
+                if (ix * ix + iy * iy + iz * iz <= ntab) ik.push_back({ix, iy, iz});
+    const int ntot = int(ik.size());
+    for (int i = 1; i < ntot; ++i)
+        for (int j = 0; j < i; ++j)
+            if (isq(ik[i]) < isq(ik[j])) std::swap(ik[i], ik[j]);
+    first.assign(1, 0);
+    count.assign(1, 1);
+    for (int i = 1; i < ntot; ++i) {
+        if (isq(ik[i]) != isq(ik[i - 1])) {
+            first.push_back(i);
+            count.push_back(0);
+        }
+        ++count.back();
+    }
+}
+
 namespace {
 
 int kshells(System &s, std::string &err) {

@@ -6,7 +6,8 @@ What follows is the host arithmetic of the reference modules:
 
 * ``updategofr`` / ``gofr_rows`` / ``rho_rows`` follow gofr.f90:100-177;
 * ``updatesofq`` / ``sofq_rows`` follow sofq.f90:67-119;
-* ``Gofr`` and ``Sofq`` keep the module state (setup, zero, add, update, write) as
+* ``updateenk`` / ``nofk_rows`` follow momentum3d.f90:149-190 (n(k), ``Engine.addenk``);
+* ``Gofr``, ``Sofq`` and ``Nofk`` keep the module state (setup, zero, add, update, write) as
   operators.f90 drives it once per block (dmcbcs.f90:465-478).
 
 Tables are written in the reference's text format (``e15.7`` columns, appended).
@@ -189,3 +190,48 @@ class Sofq:
         s, se, totalw = self.result
         _write(filename, "# q/kF, s(q), err",
                sofq_rows(s, se, self.numsh, self.ak, self.npart, self.kf), totalw, tau)
+
+
+# ---------------------------------------------------------------------------------- n(k)
+def updateenk(enknow, enkerr, wsblk):
+    """momentum3d.f90:149-171: returns (enk, enke, totalw)."""
+    norm = 1.0 / wsblk
+    e = np.asarray(enknow) * norm
+    e2 = np.asarray(enkerr) * norm
+    return e, np.sqrt(np.abs(e2 - e ** 2) / wsblk), wsblk
+
+
+def nofk_rows(enk, enke, kmag):
+    """momentum3d.f90:184-187: rows k (the magnitude of the shell's first vector; the header
+    says k/kF but the reference writes k), n(k), err."""
+    return np.stack([np.asarray(kmag), np.asarray(enk), np.asarray(enke)], axis=1)
+
+
+class Nofk:
+    """module pdistribution (momentum3d.f90) over a device population: setup, zero, add,
+    update, write."""
+
+    def __init__(self, engine, ntab):
+        self.engine, self.ntab = engine, int(ntab)
+        self.kmag, self.nk = engine.nofk_shells(self.ntab)
+        self.zero()
+
+    def zero(self):
+        n = len(self.nk)
+        self.acc = (np.zeros(n), np.zeros(n), np.zeros(1))
+        self.result = None
+
+    def add(self):
+        """addenk over every walker of the engine's population."""
+        self.engine.addenk(self.ntab, self.acc)
+
+    def update(self):
+        enknow, enkerr, ws = self.acc
+        self.result = updateenk(enknow, enkerr, float(ws[0]))
+        for a in self.acc:
+            a[...] = 0.0
+        return self.result
+
+    def write(self, filename="nofk.out", tau=0.0):
+        enk, enke, totalw = self.result
+        _write(filename, "# k/kF, n(k), err", nofk_rows(enk, enke, self.kmag), totalw, tau)

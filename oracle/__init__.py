@@ -87,8 +87,25 @@ def lib():
                                    C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p]
         L.dmco_addsofq.argtypes = [C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
                                    _f64p, _f64p, _f64p]
+        L.dmco_enk_shells.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i32p,
+                                      C.POINTER(C.c_int32)]
+        L.dmco_enk_shells.restype = C.c_int
+        L.dmco_addenk.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _i32p, _f64p, C.c_int64,
+                                  C.c_double, _f64p, _f64p, _f64p]
         _lib = L
     return _lib
+
+
+def enk_shells(ntab, maxk=8192):
+    """setupenk (momentum3d.f90:13-79): (ik [ntot][3], indk [nsh], nk [nsh])"""
+    ik = np.zeros(3 * maxk, np.int32)
+    indk = np.zeros(maxk, np.int32)
+    nk = np.zeros(maxk, np.int32)
+    ntot = C.c_int32()
+    nsh = lib().dmco_enk_shells(int(ntab), int(maxk), ik, indk, nk, C.byref(ntot))
+    if nsh < 0:
+        raise ValueError("too many n(k) vectors")
+    return ik[: 3 * ntot.value].reshape(-1, 3).copy(), indk[:nsh].copy(), nk[:nsh].copy()
 
 
 def kshells(el, nsh, maxk=4096):
@@ -242,6 +259,18 @@ class System:
     def set_params(self, vorb, bbeta, dbeta):
         self.vorb = np.ascontiguousarray(vorb, np.float64)
         self.L.dmco_set_params(self.h, self.vorb, float(bbeta), float(dbeta))
+
+    def addenk(self, configs, irns, weights, ntab):
+        """addenk (momentum3d.f90:89-120) over configurations with their RNG states and
+        weights: (enknow [nsh], enkerr [nsh], wsblk)"""
+        ik, indk, nk = enk_shells(ntab)
+        nsh = len(nk)
+        en, ee, ws = np.zeros(nsh), np.zeros(nsh), np.zeros(1)
+        for x, irn, w in zip(configs, irns, weights):
+            self.L.dmco_addenk(self.h, nsh, ik.ravel(), indk, nk,
+                               np.ascontiguousarray(x, np.float64).ravel(), int(irn), float(w),
+                               en, ee, ws)
+        return en, ee, float(ws[0])
 
     def init_walkers(self, seed, nwalk):
         x = np.zeros(nwalk * 3 * self.npart)

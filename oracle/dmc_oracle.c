@@ -1289,3 +1289,83 @@ void dmco_addsofq(int32_t npart, int32_t nup, int32_t nq, const double *ak, cons
     }
     *wsblk += weight;
 }
+
+/* momentum3d.f90:13-79 setupenk with ndim = 3: the octant vectors (ix, iy, iz >= 0,
+ * |m|^2 <= ntab) in the reference's enumeration order, put in |m|^2 order by its exchange
+ * pass, and grouped in shells of equal |m|^2: ik[ntot][3], indk[nsh] (first vector of each
+ * shell), nk[nsh].  Returns nsh, or -1 when more than maxk vectors. */
+int dmco_enk_shells(int32_t ntab, int32_t maxk, int32_t *ik, int32_t *indk, int32_t *nk,
+                    int32_t *ntot_out) {
+    int nnk = (int)sqrt((double)ntab) + 1, ntot = 0;
+    for (int ix = 0; ix <= nnk; ix++)
+        for (int iy = 0; iy <= nnk; iy++)
+            for (int iz = 0; iz <= nnk; iz++) {
+                if (ix * ix + iy * iy + iz * iz > ntab) continue;
+                if (ntot >= maxk) return -1;
+                ik[3 * ntot] = ix; ik[3 * ntot + 1] = iy; ik[3 * ntot + 2] = iz;
+                ntot++;
+            }
+    int *k2 = malloc(sizeof(int) * (ntot > 0 ? ntot : 1));
+    for (int i = 0; i < ntot; i++)
+        k2[i] = ik[3 * i] * ik[3 * i] + ik[3 * i + 1] * ik[3 * i + 1] + ik[3 * i + 2] * ik[3 * i + 2];
+    for (int i = 1; i < ntot; i++)
+        for (int j = 0; j < i; j++)
+            if (k2[i] < k2[j]) {
+                int t = k2[i]; k2[i] = k2[j]; k2[j] = t;
+                for (int d = 0; d < 3; d++) { t = ik[3 * i + d]; ik[3 * i + d] = ik[3 * j + d]; ik[3 * j + d] = t; }
+            }
+    int itab = 0;
+    indk[0] = 0;
+    nk[0] = 1;
+    for (int i = 1; i < ntot; i++) {
+        if (k2[i] != k2[i - 1]) {
+            itab++;
+            indk[itab] = i;
+            nk[itab] = 0;
+        }
+        nk[itab]++;
+    }
+    free(k2);
+    *ntot_out = ntot;
+    return itab + 1;
+}
+
+/* momentum3d.f90:89-120 addenk: for every particle i and nsamp = 10 samples, a displacement
+ * dx = (u - 1/2) L from three ran1 draws of the walker's stream (set from irn and not saved
+ * back), the ratio psirat of move1 (the inverse of hpsi, wavefunction.f90:393-404), and
+ * enktmp(shell) += psirat * prod_d cos(dk m_d dx_d) over the shell's vectors; then
+ * enknow += enktmp/nk * wt, enkerr += (enktmp/nk)^2 * wt, wsblk += wt * nsamp. */
+void dmco_addenk(const dmco_ctx *c, int32_t nsh, const int32_t *ik, const int32_t *indk,
+                 const int32_t *nk, const double *x, int64_t irn, double wt, double *enknow,
+                 double *enkerr, double *wsblk) {
+    const int nsamp = 10, n = c->n, np = c->npart;
+    double dk = 2.0 * (4.0 * atan(1.0)) / c->el;
+    double *sinv = malloc(sizeof(double) * (n * n + 3 * np)), *dphi = sinv + n * n;
+    double *tmp = calloc(nsh, sizeof(double));
+    double phil, d2phi;
+    int32_t is;
+    dmco_bcs(c, x, &phil, &is, dphi, &d2phi, sinv);
+    int64_t s = irn & (int64_t)MASK48;
+    for (int i = 0; i < np; i++)
+        for (int ii = 0; ii < nsamp; ii++) {
+            double dx[3], xnew[3];
+            for (int d = 0; d < 3; d++) {
+                dx[d] = (dmco_ran1(&s) - 0.5) * c->el;
+                xnew[d] = x[3 * i + d] + dx[d];
+            }
+            double psirat = jastrow_ratio(c, x, i, xnew) * det_ratio(c, x, sinv, i, xnew);
+            for (int k = 0; k < nsh; k++)
+                for (int j = 0; j < nk[k]; j++) {
+                    const int32_t *m = ik + 3 * (indk[k] + j);
+                    tmp[k] += psirat * cos(dk * m[0] * dx[0]) * cos(dk * m[1] * dx[1]) *
+                              cos(dk * m[2] * dx[2]);
+                }
+        }
+    for (int k = 0; k < nsh; k++) {
+        enknow[k] += tmp[k] / nk[k] * wt;
+        enkerr[k] += (tmp[k] / nk[k]) * (tmp[k] / nk[k]) * wt;
+    }
+    *wsblk += wt * nsamp;
+    free(tmp);
+    free(sinv);
+}

@@ -114,4 +114,12 @@ void dmco_addgofr(int32_t npart, int32_t nup, double el, int32_t ntab, const dou
 /* addsofq (sofq.f90:40-65): snow/serr [nq][3] */
 void dmco_addsofq(int32_t npart, int32_t nup, int32_t nq, const double *ak, const double *x,
                   double weight, double *snow, double *serr, double *wsblk);
+/* setupenk (momentum3d.f90:13-79): octant vectors ik[ntot][3] in shells indk[nsh], nk[nsh];
+ * returns nsh (-1: more than maxk vectors) */
+int dmco_enk_shells(int32_t ntab, int32_t maxk, int32_t *ik, int32_t *indk, int32_t *nk,
+                    int32_t *ntot);
+/* addenk (momentum3d.f90:89-120): enknow/enkerr [nsh] */
+void dmco_addenk(const dmco_ctx *c, int32_t nsh, const int32_t *ik, const int32_t *indk,
+                 const int32_t *nk, const double *x, int64_t irn, double wt, double *enknow,
+                 double *enkerr, double *wsblk);
 #endif

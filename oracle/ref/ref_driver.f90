@@ -19,7 +19,7 @@
 ! and then calls the reference's own setpsi/setphi/setstep/hpsi/step1/branch/
 ! move1/getderpsi/update/result unchanged.
 !
-! The observables mode also links the reference's gofr.f90 and sofq.f90.
+! The observables modes also link the reference's gofr.f90, sofq.f90 and momentum3d.f90.
 !
 ! Usage:  ref_driver <mode> <infile> <outfile>  < deck
 ! All binary files are little-endian stream files of int64 / float64 values.
@@ -37,6 +37,7 @@ program ref_driver
    use optimizer
    use gofr
    use sofq
+   use pdistribution
    implicit none
    integer, parameter :: i4=selected_int_kind(9)
    integer, parameter :: i8=selected_int_kind(15)
@@ -129,6 +130,8 @@ program ref_driver
          call do_opt
       case ('obs')
          call do_obs
+      case ('enk')
+         call do_enk
       case default
          write (6,*) 'unknown mode ',trim(mode)
          stop 1
@@ -462,5 +465,31 @@ contains
    write (21) real(nc,r8),el,kunit
    close (21)
    end subroutine do_obs
+
+! n(k) of configurations read with their RNG states and weights from infile: the
+! reference's setupenk/zeroenk/addenk/updateenk/writeenk (momentum3d.f90:13-190) unchanged,
+! as operators.f90:68,86,105,122,139 call them; the table goes to nofk.out in the working
+! directory, outfile gets the number of configurations, el and k_F.
+   subroutine do_enk
+   integer(kind=i8) :: nc,k,ntk
+   real(kind=r8) :: wt
+   call openin
+   read (22) ntk
+   read (22) nc
+   call setupenk(ndim,int(ntk,i4),el,kunit)
+   call zeroenk
+   do k=1,nc
+      call read_config(w1%x)
+      read (22) w1%irn
+      read (22) wt
+      call addenk(w1,wt)
+   enddo
+   close (22)
+   call updateenk
+   call writeenk('nofk.out',0.0_r8)
+   call openout
+   write (21) real(nc,r8),el,kunit
+   close (21)
+   end subroutine do_enk
 
 end program ref_driver

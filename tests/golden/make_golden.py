@@ -250,12 +250,37 @@ def obs_fixture(n, norb, configs, weights, ntg, ntq, tag):
              sofq=_table(texts["sofq.out"]), deck=deck)
 
 
+def enk_fixture(n, norb, configs, irns, weights, ntk, tag):
+    """n(k) of weighted configurations with their RNG states through the reference's
+    momentum3d.f90 (setupenk, addenk per configuration, updateenk, writeenk)"""
+    deck, p = refdeck.make_deck(npart=n, norb=norb)
+    nc = len(configs)
+    body = [pack([ntk], [nc])]
+    for x, irn, w in zip(configs, irns, weights):
+        body.append(pack(x, np.array([irn], np.int64), [w]))
+    data, _, texts = refrun.run("enk", deck, b"".join(body), files=("nofk.out",))
+    r = Reader(data)
+    assert int(r.f()) == nc
+    el, kf = r.f(), r.f()
+    np.savez(os.path.join(HERE, f"enk_{tag}.npz"), npart=n, norb=norb, x=configs, irn=irns,
+             w=weights, ntk=ntk, el=el, kf=kf, nofk=_table(texts["nofk.out"]), deck=deck)
+
+
 def main_obs():
     # uniform configurations with weights of both signs (is * wt0 = -1 walkers included)
     for n, norb, nc, ntg, ntq, seed in ((14, 3, 24, 40, 6, 7014), (66, 5, 12, 60, 8, 7066)):
         rng = np.random.default_rng(seed)
         w = rng.uniform(0.5, 1.5, nc) * np.where(rng.random(nc) < 0.2, -1.0, 1.0)
         obs_fixture(n, norb, random_configs(n, nc, seed), w, ntg, ntq, f"N{n}")
+
+
+def main_enk():
+    # n(k): 48-bit RNG states and weights of both signs
+    for n, norb, nc, ntk, seed in ((14, 3, 6, 20, 8014), (66, 5, 3, 12, 8066)):
+        rng = np.random.default_rng(seed)
+        w = rng.uniform(0.5, 1.5, nc) * np.where(rng.random(nc) < 0.3, -1.0, 1.0)
+        irn = rng.integers(1, 1 << 48, nc, dtype=np.int64)
+        enk_fixture(n, norb, random_configs(n, nc, seed), irn, w, ntk, f"N{n}")
 
 
 def main():
@@ -291,5 +316,7 @@ if __name__ == "__main__":
         main_vmc()
     elif "--obs" in sys.argv:
         main_obs()
+    elif "--enk" in sys.argv:
+        main_enk()
     else:
         main()

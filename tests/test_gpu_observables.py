@@ -1,10 +1,13 @@
-"""g(r), density profile and S(q) on the device (k_obs through dmc_gofr / dmc_sofq) against the
-oracle's addgofr / addsofq on the same walkers.  The oracle is pinned to the reference's own
-gofr.f90 / sofq.f90 tables (tests/test_observables.py).
+"""g(r), density profile and S(q) (k_obs through dmc_gofr / dmc_sofq) and n(k) (k_nofk
+through dmc_nofk) on the device against the oracle's addgofr / addsofq / addenk on the same
+walkers.  The oracle is pinned to the reference's own gofr.f90 / sofq.f90 / momentum3d.f90
+tables (tests/test_observables.py).
 
 The device sums one weighted count per bin and walker in another order than the reference's
 sequential loop, and S(q) forms exp(-i q.x) from per-axis powers instead of one sincos per
-(q, particle): sums agree to 1e-10 relative to the largest entry of each table.
+(q, particle): sums agree to 1e-10 relative to the largest entry of each table.  n(k) takes
+its plane-wave ratio terms through projections on the inverse and its cosines by the
+Chebyshev recurrence: 1e-9.
 """
 import numpy as np
 import pytest
@@ -93,3 +96,48 @@ def test_obs_accumulates_and_rejects_oversize():
         e.addgofr(4000)
     with pytest.raises(DMCError):
         e.addgofr(0)
+
+
+def _check_nofk(oracle_mod, e, s, ntk):
+    w = e.walkers()
+    wt = w["weight"] * w["is_"] * w["wt0"]
+    print(f"n(k): population {len(wt)}, ntab {ntk}", flush=True)
+    dev = e.addenk(ntk)
+    ref = s.addenk(w["x"], w["irn"], wt, ntk)
+    _close(dev[0], ref[0], 1e-9)
+    _close(dev[1], ref[1], 1e-9)
+    assert dev[2][0] == pytest.approx(ref[2], rel=1e-12)
+    kmag, nk = e.nofk_shells(ntk)
+    ik, indk, nk_o = oracle_mod.enk_shells(ntk)
+    np.testing.assert_array_equal(nk, nk_o)
+    np.testing.assert_allclose(kmag, 2 * np.pi / e.el * np.sqrt((ik[indk] ** 2).sum(axis=1)),
+                               rtol=1e-15)
+
+
+def test_nofk_dmc_population_n14(oracle_mod):
+    """addenk (momentum3d.f90:89-120) of a DMC population on the device (k_nofk: FP64
+    inverse, projections of the partner phases, one lane per displacement) against the
+    oracle on the read-back walkers, RNG states and weights."""
+    from dmcbs_project_amd import Engine
+    e = Engine(npart=14, norb=3, nwalk=400, fn=False, dt=0.01)
+    e.init_walkers(31)
+    for _ in range(10):
+        e.step()
+    _check_nofk(oracle_mod, e, oracle_mod.System(npart=14, norb=3), 20)
+
+
+def test_nofk_reference_configurations_n66(oracle_mod):
+    """The configurations and RNG states of the reference's n(k) table at N = 66, loaded as
+    walkers (weight 1), and a short DMC population: device sums equal the oracle's."""
+    from dmcbs_project_amd import Engine, DMC_MIXED, DMC_SINGLEPARTICLE
+    g = golden("enk_N66.npz")
+    s = oracle_mod.System(npart=66, norb=5)
+    e = Engine(npart=66, norb=5, nwalk=len(g["x"]))
+    e.push_walkers(g["x"], g["irn"])
+    _check_nofk(oracle_mod, e, s, int(g["ntk"]))
+    e = Engine(npart=66, norb=5, nwalk=96, dt=0.001, dcut=2.0, capacity=4 * 96,
+               precision=DMC_MIXED, movetype=DMC_SINGLEPARTICLE, krefresh=33)
+    e.init_walkers(5)
+    for _ in range(2):
+        e.step()
+    _check_nofk(oracle_mod, e, s, 20)

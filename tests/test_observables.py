@@ -83,3 +83,53 @@ def test_fortran_e_format():
     assert obs.fortran_e(0.0) == "  0.0000000E+00"
     assert obs.fortran_e(1.0) == "  0.1000000E+01"
     assert len(obs.fortran_e(math.pi * 1e-30)) == 15
+
+
+@pytest.mark.parametrize("n,norb", [(14, 3), (66, 5)])
+def test_nofk_vs_reference(oracle_mod, n, norb):
+    """addenk (momentum3d.f90:89-120: ten uniform displacements per particle from the
+    walker's ran1 stream, move1 ratios, cosine products per octant shell) against the
+    reference's nofk.out for configurations with RNG states and weights of both signs."""
+    g = golden(f"enk_N{n}.npz")
+    s = oracle_mod.System(npart=n, norb=norb)
+    assert s.el == float(g["el"])
+    ntk = int(g["ntk"])
+    en, ee, ws = s.addenk(g["x"], g["irn"], g["w"], ntk)
+    assert ws == pytest.approx(10 * float(np.sum(g["w"])), rel=1e-14)
+    ik, indk, nk = oracle_mod.enk_shells(ntk)
+    kmag = 2 * math.pi / s.el * np.sqrt((ik[indk] ** 2).sum(axis=1))
+    enk, enke, _ = obs.updateenk(en, ee, ws)
+    _close_printed(obs.nofk_rows(enk, enke, kmag), g["nofk"])
+
+
+def test_nofk_shells_and_closed_forms(oracle_mod):
+    """setupenk's shells are every octant vector with |m|^2 <= ntab once, in |m|^2 order; the
+    k = 0 shell of addenk is the plain sum of the walker's move1 ratios over its 10 N
+    displacements, drawn from its ran1 stream (the LCG closed form here), in the order
+    particle, sample, axis."""
+    ik, indk, nk = oracle_mod.enk_shells(20)
+    k2 = (ik ** 2).sum(axis=1)
+    assert np.all(np.diff(k2) >= 0) and len(ik) == nk.sum() == len(set(map(tuple, ik)))
+    brute = {(a, b, c) for a in range(5) for b in range(5) for c in range(5)
+             if a * a + b * b + c * c <= 20}
+    assert set(map(tuple, ik)) == brute
+    assert np.all(k2[indk] == np.unique(k2))
+    g = golden("enk_N14.npz")
+    s = oracle_mod.System(npart=14, norb=3)
+    x, irn = g["x"][0], int(g["irn"][0])
+    en, _, _ = s.addenk([x], [irn], [1.0], 0)
+    state, tot = irn, 0.0
+    for i in range(14):
+        for _ in range(10):
+            u = []
+            for _ in range(3):
+                v, state = _ran1(state)
+                u.append(v)
+            tot += s.ratio1(x, i, x[i] + (np.array(u) - 0.5) * s.el)
+    assert en[0] == pytest.approx(tot, rel=1e-12)
+
+
+def _ran1(state):
+    """ran1 (ran.f90:8-27) as the 48-bit LCG closed form"""
+    state = (state * 44485709377909 + 96309754297) % (1 << 48)
+    return (state | 1) * 2.0 ** -48, state
