@@ -226,7 +226,7 @@ def batch_run(steps, warmup, tune=True):
     return res
 
 
-def observables_run(steps, warmup, ntab=100, ntabq=10):
+def observables_run(steps, warmup, ntab=100, ntabq=10, ntabk=20):
     """g(r) with density profile (addgofr, ntab bins) and S(q) (addsofq, ntabq shells) over
     the resident headline population after 3 mixed-precision steps: one "step" is one block's
     operator pass over every walker (dmcbcs.f90:465-471), host sums included."""
@@ -245,8 +245,17 @@ def observables_run(steps, warmup, ntab=100, ntabq=10):
     e.sync()
     dt = time.perf_counter() - t0
     nw = e.numstack()
+    # n(k) (addenk, 20 shells of |m|^2 <= 20): 660 single-particle ratios per walker
+    kacc = e.addenk(ntabk)
+    e.sync()
+    t1 = time.perf_counter()
+    for _ in range(max(1, steps // 5)):
+        e.addenk(ntabk, kacc)
+    e.sync()
+    dtk = (time.perf_counter() - t1) / max(1, steps // 5)
     return dict(value=steps * nw / dt, ms_per_step=1e3 * dt / steps, population=nw,
-                nq=len(e.sofq_kvec(ntabq)[1]))
+                nq=len(e.sofq_kvec(ntabq)[1]), nofk_ms=1e3 * dtk,
+                nofk_ratios_per_s=nw * N * 10 / dtk)
 
 
 def traffic_from_profiles(workload):
@@ -373,7 +382,9 @@ def main():
             "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic: reference initial walkers after 3 DMC steps",
             "config": {"workload": f"observables N={N} {r['population']} walkers, "
-                                   f"ntabg=100, ntabq=10 ({r['nq']} q vectors)"}}), flush=True)
+                                   f"ntabg=100, ntabq=10 ({r['nq']} q vectors)"},
+            "nofk": {"ms_per_pass": r["nofk_ms"], "ntab": 20,
+                     "ratios_per_s": r["nofk_ratios_per_s"]}}), flush=True)
         return
     main_run = timed_run(a.workload, a.steps, a.warmup)
     print(f"bench: {a.workload}: {main_run}", file=sys.stderr, flush=True)

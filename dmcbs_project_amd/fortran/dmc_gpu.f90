@@ -149,6 +149,20 @@ module dmc_gpu
          integer(c_int32_t), value :: ntabq
          real(c_double), intent(inout) :: snow(*), serr(*), wsblk
       end function
+      integer(c_int) function dmc_nofk_shells(ctx, ntab, cap, nsh, kmag, nk) &
+            bind(c, name='dmc_nofk_shells')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ntab, cap
+         integer(c_int32_t), intent(out) :: nsh
+         type(c_ptr), value :: kmag, nk
+      end function
+      integer(c_int) function dmc_nofk(ctx, ntab, enknow, enkerr, wsblk) bind(c, name='dmc_nofk')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ntab
+         real(c_double), intent(inout) :: enknow(*), enkerr(*), wsblk
+      end function
       integer(c_int) function dmc_sync(ctx) bind(c, name='dmc_sync')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

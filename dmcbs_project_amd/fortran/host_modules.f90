@@ -8,9 +8,10 @@
 !   step         setstep, step1, branch, writewtisto         (step.f90:11-334)
 !   estimator    setestnum, addest, zerest, update, resstring (estimator.f90:19-149)
 !   optimizer    setupopt, calder, updatepar                  (optimizer.f90:14-169)
-!   gofr, sofq   setup, zero, add, update, write              (gofr.f90:14-177, sofq.f90:14-119)
-!   operators    setoperators, zero/add/update/writeoperators (operators.f90:7-140, g(r)
-!                and S(q) only)
+!   gofr, sofq, pdistribution   setup, zero, add, update, write of g(r), S(q), n(k)
+!                (gofr.f90:14-177, sofq.f90:14-119, momentum3d.f90:13-190)
+!   operators    setoperators, zero/add/update/writeoperators (operators.f90:7-140, g(r),
+!                S(q) and n(k))
 ! All computation runs on the device through the C-ABI (module dmc_gpu); the walker
 ! population stays in device memory.  The deck values each reference module reads from
 ! unit 5 are read here in the same order and collected into one dmc_params record, and the
@@ -699,18 +700,100 @@ contains
    end subroutine writesofq
 end module sofq
 
-! operators.f90:7-140 for a periodic box: the four table sizes of the deck; g(r) and S(q)
-! are computed, n(k) and the euclidean density response stop the run.  The walker loops of
+! n(k) (momentum3d.f90:13-190): the shells of setupenk come from the engine
+! (dmc_nofk_shells) at the first zero or add, the sums of every walker from one call of
+! dmc_nofk.
+module pdistribution
+   use engine
+   implicit none
+   private
+   public :: setupenk, zeroenk, addenk_population, updateenk, writeenk
+   integer(kind=i4), save :: ntab, nsh
+   logical, save :: ready=.false.
+   real(kind=r8), save :: wsblk, totalw
+   real(c_double), save, allocatable, target :: kmag(:)
+   integer(c_int32_t), save, allocatable, target :: nk(:)
+   real(kind=r8), save, allocatable :: enknow(:), enkerr(:), enk(:), enke(:)
+contains
+   subroutine setupenk(ndim,ntabin,elin,kfin)
+   integer(kind=i4) :: ndim,ntabin
+   real(kind=r8) :: elin,kfin
+   if (ndim.ne.3) error stop 'setupenk: three dimensions only'
+   ntab=ntabin
+   end subroutine setupenk
+
+   subroutine shells
+   integer(c_int32_t) :: n
+   if (ready) return
+   call dmc_check(dmc_nofk_shells(engine_ctx(),ntab,0,n,c_null_ptr,c_null_ptr), &
+      'dmc_nofk_shells')
+   nsh=n
+   allocate(kmag(nsh),nk(nsh),enknow(nsh),enkerr(nsh),enk(nsh),enke(nsh))
+   call dmc_check(dmc_nofk_shells(engine_ctx(),ntab,n,n,c_loc(kmag),c_loc(nk)), &
+      'dmc_nofk_shells')
+   enknow=0.0_r8
+   enkerr=0.0_r8
+   enk=0.0_r8
+   enke=0.0_r8
+   wsblk=0.0_r8
+   ready=.true.
+   end subroutine shells
+
+   subroutine zeroenk
+   call shells
+   enk=0.0_r8
+   enknow=0.0_r8
+   enkerr=0.0_r8
+   enke=0.0_r8
+   wsblk=0.0_r8
+   end subroutine zeroenk
+
+   subroutine addenk_population
+   call shells
+   call dmc_check(dmc_nofk(engine_ctx(),ntab,enknow,enkerr,wsblk),'dmc_nofk')
+   end subroutine addenk_population
+
+   subroutine updateenk
+   real(kind=r8) :: norm
+   totalw=wsblk
+   norm=1.0_r8/wsblk
+   enk=enknow*norm
+   enke=sqrt(abs(enkerr*norm-enk**2)/wsblk)
+   wsblk=0.0_r8
+   enknow=0.0_r8
+   enkerr=0.0_r8
+   end subroutine updateenk
+
+   subroutine writeenk(filenamek,tau)
+   character(len=*) :: filenamek
+   real(kind=r8) :: tau
+   integer(kind=i4) :: ir
+   open (unit=78,file=filenamek,position='append')
+   write (78,'(''# k/kF, n(k), err'')')
+   write (78,'(''# total weight = '',e15.7)') totalw
+   write (78,'(''# tau = '',e15.7)') tau
+   do ir=1,nsh
+      write (78,'(6e15.7)') kmag(ir),enk(ir),enke(ir)
+   enddo
+   write (78,*) ''
+   write (78,*) ''
+   close (78)
+   end subroutine writeenk
+end module pdistribution
+
+! operators.f90:7-140 for a periodic box: the four table sizes of the deck; g(r), S(q) and
+! n(k) are computed, the euclidean density response stops the run.  The walker loops of
 ! dmcbcs.f90:385-390,465-471 (pop, addoperators, push) become one call per observable over
 ! the device population.
 module operators
    use engine
    use gofr
    use sofq
+   use pdistribution
    implicit none
    private
    public :: setoperators, zerooperators, addoperators, updateoperators, writeoperators
-   logical, save :: igofr=.false., isofq=.false.
+   logical, save :: igofr=.false., isofq=.false., ienk=.false.
 contains
    subroutine setoperators(ndim,nbin,el,oper,kunit)
    integer(kind=i4) :: ndim,nbin(:)
@@ -722,33 +805,39 @@ contains
    read (5,*) ntabq
    read (5,*) ntabenk
    read (5,*) ntabqeuc
-   if (ntabenk.gt.0.or.ntabqeuc.gt.0) &
-      error stop 'setoperators: n(k) and the euclidean density response are not supported'
+   if (ntabqeuc.gt.0) &
+      error stop 'setoperators: the euclidean density response is not supported'
    igofr=ntabg.gt.0
    isofq=ntabq.gt.0
+   ienk=ntabenk.gt.0
    if (igofr) call setupgofr(ndim,ntabg,nbin,el,kunit)
    if (isofq) call setupsofq(ndim,nbin,el,ntabq,kunit)
-   oper=igofr.or.isofq
+   if (ienk) call setupenk(ndim,ntabenk,el,kunit)
+   oper=igofr.or.isofq.or.ienk
    end subroutine setoperators
 
    subroutine zerooperators
    if (igofr) call zerogofr
    if (isofq) call zerosofq
+   if (ienk) call zeroenk
    end subroutine zerooperators
 
    subroutine addoperators
    if (igofr) call addgofr_population
    if (isofq) call addsofq_population
+   if (ienk) call addenk_population
    end subroutine addoperators
 
    subroutine updateoperators
    if (igofr) call updategofr
    if (isofq) call updatesofq
+   if (ienk) call updateenk
    end subroutine updateoperators
 
    subroutine writeoperators(tau)
    real(kind=r8) :: tau
    if (igofr) call writegofr('gofr.out',tau)
    if (isofq) call writesofq('sofq.out',tau)
+   if (ienk) call writeenk('nofk.out',tau)
    end subroutine writeoperators
 end module operators

@@ -168,6 +168,19 @@ int dmc_sofq_kvec(dmc_ctx *c, int32_t ntabq, int32_t cap, int32_t *nqtot, int32_
  *                          the real parts of their squares * weight
  *   wsblk                  the summed weight */
 int dmc_sofq(dmc_ctx *c, int32_t ntabq, double *snow, double *serr, double *wsblk);
+/* setupenk (momentum3d.f90:13-79): the shells of octant vectors m (m_x, m_y, m_z >= 0,
+ * |m|^2 <= ntab, 0 <= ntab <= 63) of n(k): *nsh shells, kmag[nsh] = 2 pi |m| / L of each
+ * shell's first vector (the first column of writeenk, momentum3d.f90:186), nk[nsh] vectors
+ * per shell; kmag and nk may be NULL (then only *nsh), else cap >= nsh. */
+int dmc_nofk_shells(dmc_ctx *c, int32_t ntab, int32_t cap, int32_t *nsh, double *kmag,
+                    int32_t *nk);
+/* addenk (momentum3d.f90:89-120) for every walker of the resident population: ten uniform
+ * displacements per particle from the walker's own ran1 stream (irn is read, not advanced),
+ * the move1 ratio (wavefunction.f90:393-404) of each, and per shell k
+ *   enknow[k] += enktmp_k/nk_k * weight,  enkerr[k] += (enktmp_k/nk_k)^2 * weight,
+ *   wsblk += 10 * weight,
+ * with enktmp_k = sum psirat * prod_d cos(2 pi m_d dx_d / L) over the shell's vectors. */
+int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *wsblk);
 
 /* Timing of the dominant kernel with HIP events recorded on the stream it is launched on:
  * the move kernel (k_sm_moves, every chunk of krefresh moves) for single-particle moves, the

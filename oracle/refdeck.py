@@ -21,6 +21,7 @@ DEFAULTS = dict(
     rho=0.0337737278, norb=3, vorb=None, bbeta=0.5, dbeta=5.0, bfact=1.0, v0=1.0,
     amu=40.0, cjas1=1.0, cjas2=0.0, djas=1.15, v0op=0.0, v0eq=0.0,
     optiter=1, optjas=False, optnodes=True, delta=0.7, eps=1.0e-4, ntabg=0, ntabq=0,
+    ntabk=0,
 )
 
 
@@ -48,7 +49,7 @@ def make_deck(**kw):
         repr(float(p["bbeta"])), repr(float(p["dbeta"])), repr(float(p["bfact"])),
         repr(float(p["v0"])), repr(float(p["amu"])), repr(float(p["cjas1"])),
         repr(float(p["cjas2"])), repr(float(p["djas"])), repr(float(p["v0op"])),
-        repr(float(p["v0eq"])), str(p["ntabg"]), str(p["ntabq"]), "0", "0",
+        repr(float(p["v0eq"])), str(p["ntabg"]), str(p["ntabq"]), str(p["ntabk"]), "0",
     ]
     if p["idmc"] in (-1, -2):
         lines += [str(p["optiter"]), _l(p["optjas"]), _l(p["optnodes"]),
