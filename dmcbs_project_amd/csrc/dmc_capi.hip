@@ -254,6 +254,8 @@ int sofq_vectors(dmc_ctx *c, int ntabq) {
 constexpr size_t kObsLdsMax = 64 * 1024;
 // n(k) shells up to |m|^2 = 63: every component <= 7, the kernel's cosine table
 constexpr int kNofkMaxTab = 63;
+// walkers per n(k) batch: bounds the per-walker buffers (37 KB per walker at N = 66)
+constexpr int kNofkBatch = 8192;
 
 // g(r)/density (ntab > 0) or S(q) (ntabq > 0) sums of the resident population, in the
 // entry order of ObsArgs
@@ -828,11 +830,12 @@ int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *w
     std::vector<int> first, count;
     enk_shells(ntab, ik, first, count);
     const int nsh = int(count.size()), ntot = int(ik.size()), nw = c->nw, np = c->sys.npart;
+    const int nup = c->sys.n, nq = c->ds.nq;
     const int nsamp = 10;  // momentum3d.f90:22
-    const int ncol = 1 + 2 * nsh, nmove = np * nsamp;
+    const int ncol = 1 + 2 * nsh, nmove = np * nsamp, nchunk = (nmove + 63) / 64;
     if (nw == 0) return DMC_OK;
-    if (nofk_lds_bytes(c->sys.n, c->ds.nq, nsh) > kObsLdsMax)
-        return fail(DMC_ERR_ARG, "too many n(k) shells for the kernel's LDS");
+    if (nofk_prep_lds_bytes(nup, nq) > kObsLdsMax)
+        return fail(DMC_ERR_ARG, "too many plane waves for the n(k) kernel's LDS");
     // state after 3 m draws of ran1, m < npart nsamp
     std::vector<uint64_t> ja(nmove), jc(nmove);
     {
@@ -849,17 +852,27 @@ int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *w
     for (int v = 0; v < ntot; ++v) hik[v] = make_int4(ik[v][0], ik[v][1], ik[v][2], 0);
     std::vector<int2> hsh(nsh);
     for (int k = 0; k < nsh; ++k) hsh[k] = make_int2(first[k], count[k]);
+    // per-walker buffers for batches of at most kNofkBatch walkers
+    const int batch = std::min(nw, kNofkBatch);
     const int seg = 256, nseg = (nw + seg - 1) / seg;
-    const size_t nd = (size_t(nw + nseg + 1) * ncol + 1) & ~size_t(1);
-    const size_t bytes = sizeof(double) * nd + 2 * sizeof(uint64_t) * nmove +
-                         sizeof(int4) * ntot + sizeof(int2) * nsh + 256;
+    const size_t nrows = (size_t(nw + nseg + 1) * ncol + 1) & ~size_t(1);
+    const size_t nproj = size_t(batch) * np * nq * 2, ncoef = size_t(batch) * np * nup,
+                 nuold = size_t(batch) * np,
+                 npart = (size_t(batch) * nchunk * nsh + 1) & ~size_t(1);  // keeps int4 aligned
+    const size_t bytes = sizeof(double) * (nrows + nproj + ncoef + nuold + npart) +
+                         sizeof(int) * (size_t(batch) + 4) + 2 * sizeof(uint64_t) * nmove +
+                         sizeof(int4) * ntot + sizeof(int2) * nsh + 512;
     int e = c->ensure_scratch(bytes);
     if (e) return e;
     double *rows = static_cast<double *>(c->scratch), *tmp = rows + size_t(nw) * ncol,
            *out = tmp + size_t(nseg) * ncol;
-    uint64_t *dja = reinterpret_cast<uint64_t *>(rows + nd), *djc = dja + nmove;
+    double2 *proj = reinterpret_cast<double2 *>(rows + nrows);
+    double *coef = reinterpret_cast<double *>(proj) + nproj, *uold = coef + ncoef,
+           *prt = uold + nuold;
+    uint64_t *dja = reinterpret_cast<uint64_t *>(prt + npart), *djc = dja + nmove;
     int4 *dik = reinterpret_cast<int4 *>(djc + nmove);
     int2 *dsh = reinterpret_cast<int2 *>(dik + ntot);
+    int *sing = reinterpret_cast<int *>(dsh + nsh);
     HIPCHK(hipMemcpyAsync(dja, ja.data(), 8 * nmove, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(djc, jc.data(), 8 * nmove, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dik, hik.data(), sizeof(int4) * ntot, hipMemcpyHostToDevice, c->st));
@@ -870,16 +883,25 @@ int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *w
     a.wt0 = c->A.wt0;
     a.is = c->A.is;
     a.irn = c->A.irn;
-    a.nw = nw;
     a.nsamp = nsamp;
     a.nsh = nsh;
+    a.nchunk = nchunk;
     a.ik = dik;
     a.shell = dsh;
     a.ja = dja;
     a.jc = djc;
     a.dk = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;
+    a.proj = proj;
+    a.coef = coef;
+    a.uold = uold;
+    a.sing = sing;
+    a.part = prt;
     a.rows = rows;
-    LAUNCHCHK(launch_nofk(c->sys.n, c->ds, a, c->st));
+    for (int w0 = 0; w0 < nw; w0 += batch) {
+        a.w0 = w0;
+        a.nw = std::min(batch, nw - w0);
+        LAUNCHCHK(launch_nofk(nup, c->ds, a, c->st));
+    }
     LAUNCHCHK(launch_rows_reduce(rows, nw, ncol, seg, tmp, out, c->st));
     std::vector<double> sums(ncol);
     HIPCHK(hipMemcpyAsync(sums.data(), out, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->st));

@@ -82,20 +82,29 @@ int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st);
 int launch_rows_reduce(const double *in, int nrow, int ncol, int seg, double *tmp, double *out,
                        hipStream_t st);
 
-// n(k) of a walker population (addenk): one row per walker [nw][1 + 2 nsh] =
-// (wt nsamp, enktmp/nk wt [nsh], (enktmp/nk)^2 wt [nsh])
+// n(k) of walkers w0 .. w0 + nw - 1 of a population (addenk): one row per walker
+// rows[w][1 + 2 nsh] = (wt nsamp, enktmp/nk wt [nsh], (enktmp/nk)^2 wt [nsh]).  Three
+// launches: per walker the FP64 inverse, the projections of the partner phases, the partner
+// coefficients and the old Jastrow sums into per-walker buffers; then one wave per 64
+// displacements of a walker (the shell sums of each wave into part); then per walker the
+// sums over its waves and its row.
 struct NofkArgs {
     const double *x, *weight, *wt0;
     const int *is;
     const uint64_t *irn;
-    int nw, nsamp, nsh;
+    int w0, nw, nsamp, nsh, nchunk;
     const int4 *ik;          // octant vectors in shell order
     const int2 *shell;       // (first vector, count) of each shell
     const uint64_t *ja, *jc; // ran1 state after 3 m draws = ja[m] s + jc[m], m < npart nsamp
     double dk;               // 2 pi / L
-    double *rows;
+    double2 *proj;           // [nw][npart][nq] sum_j Sinv exp(i q.r_j) over the partners of i
+    double *coef;            // [nw][npart][npart/2] Sinv of i with each partner
+    double *uold;            // [nw][npart] Jastrow sum of i at its position
+    int *sing;               // [nw] singular pairing matrix
+    double *part;            // [nw][nchunk][nsh] shell sums of each wave
+    double *rows;            // [w0 + nw][1 + 2 nsh]
 };
-size_t nofk_lds_bytes(int nup, int nq, int nsh);
+size_t nofk_prep_lds_bytes(int nup, int nq);
 int launch_nofk(int nup, const DevSys &S, const NofkArgs &a, hipStream_t st);
 
 }  // namespace dmc
