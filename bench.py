@@ -322,6 +322,53 @@ def cpu_reference_baseline(seconds=20.0):
                        f"thread): {cores} processes x {nwalk} walkers x {nstep} steps at N=66")
 
 
+def cpu_reference_observables(cores=None):
+    """The reference's own addgofr + addsofq (ntabg = 100, ntabq = 10) and addenk (ntab = 20)
+    through oracle/_ref/ref_driver's obs and enk modes on the host cores, one single-threaded
+    process per core on a bounded sample of uniform N=66 configurations (walkers/s, set-up
+    included)."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    from oracle import refdeck
+    cores = cores or min(16, len(os.sched_getaffinity(0)))
+    deck, _ = refdeck.make_deck(npart=N, norb=NORB)
+    el = (N / 0.0337737278) ** (1.0 / 3.0)
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    out = {}
+    for mode, head, nc in (("obs", [100, 10], 3000), ("enk", [20], 48)):
+        rng = np.random.default_rng(SEED)
+        parts = [np.array(head + [nc], "<i8").tobytes()]
+        for k in range(nc):
+            parts.append((el * (rng.random((N, 3)) - 0.5)).astype("<f8").tobytes())
+            if mode == "enk":
+                parts.append(np.array([1 + 7919 * k], "<i8").tobytes())
+            parts.append(np.array([1.0], "<f8").tobytes())
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "in.bin")
+            with open(fin, "wb") as fh:
+                fh.write(b"".join(parts))
+            dirs = [os.path.join(td, f"p{i}") for i in range(cores)]
+            for d in dirs:
+                os.mkdir(d)
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([drv, mode, fin, os.path.join(d, "o.bin")],
+                                      stdin=subprocess.PIPE, stdout=subprocess.DEVNULL, cwd=d,
+                                      env=env) for d in dirs]
+            for p in procs:
+                p.stdin.write(deck.encode())
+                p.stdin.close()
+            for p in procs:
+                p.wait(timeout=600)
+            wall = time.perf_counter() - t0
+        out[mode] = dict(value=cores * nc / wall, unit="walkers/s", cores=cores,
+                         kind="reference",
+                         sample=f"reference {'addgofr + addsofq' if mode == 'obs' else 'addenk'}"
+                                f" (amdflang -O2): {cores} processes x {nc} configurations "
+                                f"at N={N}")
+    return out
+
+
 def cpu_port_baseline():
     """The oracle's FP64 single-particle-move sweep (the workload's algorithm) on all host
     cores (OpenMP), bounded sample."""
@@ -375,6 +422,7 @@ def main():
         return
     if a.workload == "observables":
         r = observables_run(a.steps, a.warmup)
+        cpu = None if (a.no_cpu or a.quick) else cpu_reference_observables()
         print(json.dumps({
             "metric": "walkers/sec through g(r) + density (100 bins) and S(q) (10 shells)",
             "value": r["value"], "unit": "walkers/s", "n_gpus": 1, "steps": a.steps,
@@ -383,8 +431,11 @@ def main():
             "data": "synthetic: reference initial walkers after 3 DMC steps",
             "config": {"workload": f"observables N={N} {r['population']} walkers, "
                                    f"ntabg=100, ntabq=10 ({r['nq']} q vectors)"},
+            "cpu_baseline": cpu["obs"] if cpu else None,
             "nofk": {"ms_per_pass": r["nofk_ms"], "ntab": 20,
-                     "ratios_per_s": r["nofk_ratios_per_s"]}}), flush=True)
+                     "walkers_per_s": r["population"] / (1e-3 * r["nofk_ms"]),
+                     "ratios_per_s": r["nofk_ratios_per_s"],
+                     "cpu_baseline": cpu["enk"] if cpu else None}}), flush=True)
         return
     main_run = timed_run(a.workload, a.steps, a.warmup)
     print(f"bench: {a.workload}: {main_run}", file=sys.stderr, flush=True)
