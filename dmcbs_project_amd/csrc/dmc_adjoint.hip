@@ -83,11 +83,13 @@ __global__ __launch_bounds__(64) void k_adjoint(DevSys S, const double *__restri
                 if (rj < S.range) {
                     double w0, w1, w2, w3;
                     const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
-                    const double t = lookup4(S.jt, idx, w0, w1, w2, w3).z / (rj * S.sqrtm);
+                    if (idx < S.jfar) {  // u' is zero on stencils past the healing distance
+                        const double t = lookup4(S.jt, idx, w0, w1, w2, w3).z / (rj * S.sqrtm);
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        ju[a][d] += dx[d] * t;
-                        jd[b][d] -= dx[d] * t;
+                        for (int d = 0; d < 3; ++d) {
+                            ju[a][d] += dx[d] * t;
+                            jd[b][d] -= dx[d] * t;
+                        }
                     }
                 }
             }

@@ -119,12 +119,15 @@ int upload_tables(dmc_ctx *c) {
     const System &s = c->sys;
     const int n1 = s.ntab + 1;
     std::vector<double4> jt(n1), bt(n1), btb(n1), btd(n1);
-    std::vector<double> bt1(n1);
+    std::vector<double> bt1(n1), vt1(n1);
     std::vector<double2> bt2(n1);
+    int last = -1;  // last node with a nonzero u, u' or u''
     std::vector<double4> mt(n1);
     std::vector<float4> mt32(n1);
     for (int i = 0; i < n1; ++i) {
         jt[i] = make_double4(s.vt[i], s.ut[i], s.dut[i], s.d2ut[i]);
+        vt1[i] = s.vt[i];
+        if (s.ut[i] != 0.0 || s.dut[i] != 0.0 || s.d2ut[i] != 0.0) last = i;
         bt[i] = make_double4(s.bt[i], s.dbt[i], s.d2bt[i], 0.0);
         bt1[i] = s.bt[i];
         bt2[i] = make_double2(s.dbt[i], s.d2bt[i]);
@@ -145,6 +148,9 @@ int upload_tables(dmc_ctx *c) {
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(const_cast<double *>(c->ds.bt1), bt1.data(), sizeof(double) * n1,
                           hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(const_cast<double *>(c->ds.vt1), vt1.data(), sizeof(double) * n1,
+                          hipMemcpyHostToDevice, c->st));
+    c->ds.jfar = last + 2;
     HIPCHK(hipMemcpyAsync(const_cast<double2 *>(c->ds.bt2), bt2.data(), sizeof(double2) * n1,
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(const_cast<double4 *>(c->ds.mt64), mt.data(), sizeof(double4) * n1,
@@ -372,14 +378,14 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     fill_devsys(c);
     const int n1 = s.ntab + 1, nq = s.nk - 1;
     double4 *jt, *bt, *btb, *btd, *mt64;
-    double *bt1;
+    double *bt1, *vt1;
     double2 *bt2;
     float4 *mt32;
     int4 *qm;
     double *wq;
     uint64_t *ja1, *jc1, *ja2, *jc2;
     if (c->alloc(&jt, n1) || c->alloc(&bt, n1) || c->alloc(&btb, n1) || c->alloc(&btd, n1) ||
-        c->alloc(&bt1, n1) || c->alloc(&bt2, n1) ||
+        c->alloc(&bt1, n1) || c->alloc(&bt2, n1) || c->alloc(&vt1, n1) ||
         c->alloc(&mt64, n1) || c->alloc(&mt32, n1) ||
         c->alloc(&qm, nq) || c->alloc(&wq, nq) || c->alloc(&ja1, kMaxDraw) ||
         c->alloc(&jc1, kMaxDraw) || c->alloc(&ja2, kMaxDraw) || c->alloc(&jc2, kMaxDraw) ||
@@ -396,6 +402,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     c->ds.jt = jt;
     c->ds.bt = bt;
     c->ds.bt1 = bt1;
+    c->ds.vt1 = vt1;
     c->ds.bt2 = bt2;
     c->ds.btb = btb;
     c->ds.btd = btd;

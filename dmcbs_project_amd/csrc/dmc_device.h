@@ -15,6 +15,11 @@ struct DevSys {
     double alpha0, sigma, sig2, sqrtm, invsqrtm, invm, invmu, hbar, dt, etrial, wtcut, dcut,
         nfac, twopi_l, popup, popdn;
     const double4 *jt;    // Jastrow/potential records {v, u, du, d2u}, [ntab+1]
+    // the potential alone, [ntab+1], and the first interval whose 4-point stencil
+    // (jfar - 1 .. jfar + 2 and beyond) holds u = u' = u'' = 0 exactly: pairs past the
+    // Jastrow's healing distance read 8 bytes per node instead of the 32-byte record
+    const double *vt1;
+    int jfar;
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
     // the same beta values in compact arrays, so that a pair loop touches only the fields it
     // reads (a smaller working set in the vector L1): b, [ntab+1]; {db, lap b}, [ntab+1]

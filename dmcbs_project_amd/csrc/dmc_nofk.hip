@@ -44,6 +44,7 @@ __device__ __forceinline__ double jastrow_u(const DevSys &S, double r2) {
     if (rj >= S.range) return 0.0;
     double w0, w1, w2, w3;
     const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+    if (idx >= S.jfar) return 0.0;  // u is zero on stencils past the healing distance
     return w0 * S.jt[idx - 1].y + w1 * S.jt[idx].y + w2 * S.jt[idx + 1].y +
            w3 * S.jt[idx + 2].y;
 }

@@ -636,16 +636,23 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
             if (rj < S.range) {
                 double w0, w1, w2, w3;
                 const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
-                const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
-                V += jv.x;
-                U += jv.y;
-                const double t = jv.z * irj * S.invsqrtm;
+                if (idx >= S.jfar) {
+                    // past the healing distance u, u' and u'' are zero on the whole stencil:
+                    // the potential alone, the same arithmetic as the record's first field
+                    V += w0 * S.vt1[idx - 1] + w1 * S.vt1[idx] + w2 * S.vt1[idx + 1] +
+                         w3 * S.vt1[idx + 2];
+                } else {
+                    const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
+                    V += jv.x;
+                    U += jv.y;
+                    const double t = jv.z * irj * S.invsqrtm;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    pu[a][d] += dx[d] * t;
-                    pdn[b][d] -= dx[d] * t;
+                    for (int d = 0; d < 3; ++d) {
+                        pu[a][d] += dx[d] * t;
+                        pdn[b][d] -= dx[d] * t;
+                    }
+                    D2U += (jv.w + 2.0 * jv.z * irj) * S.invmu;
                 }
-                D2U += (jv.w + 2.0 * jv.z * irj) * S.invmu;
             }
             // a scheduling fence per pair: the table loads of one pair in flight at a time
             // (unfenced, the straight-line loop issues every pair's loads at once)
@@ -950,8 +957,10 @@ __device__ double jastrow_logratio(const DevSys &S, const double *xs, int i, con
             if (rj < S.range) {
                 double w0, w1, w2, w3;
                 const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
-                const double u = lookup4(S.jt, idx, w0, w1, w2, w3).y;
-                du += pass ? u : -u;
+                if (idx < S.jfar) {  // u is zero on stencils past the healing distance
+                    const double u = lookup4(S.jt, idx, w0, w1, w2, w3).y;
+                    du += pass ? u : -u;
+                }
             }
         }
     }
