@@ -402,8 +402,9 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (walkers shard "
-              "trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the walkers "
+              "would shard across GPUs by population, each GPU stepping its own share; only "
+              "the one-GPU share is built and measured)", file=sys.stderr)
         sys.exit(2)
     print(f"bench: workload={a.workload} N={N} walkers={NWALK} steps={a.steps} "
           f"warmup={a.warmup}", file=sys.stderr, flush=True)

@@ -7,7 +7,8 @@
  *
  * Each entry point names the reference interface it replaces (file:line in the DMCBS
  * tree).  The reference exposes Fortran module procedures; its Fortran host binds these
- * functions through iso_c_binding (see INTEGRATION.md for the interface blocks).
+ * functions through iso_c_binding interface blocks (dmcbs_project_amd/fortran/dmc_gpu.f90
+ * holds one for every entry point).
  */
 #ifndef DMC_GPU_H
 #define DMC_GPU_H

@@ -92,8 +92,8 @@ int dmco_branch(int32_t nw, int32_t npart, const double *x, const double *dpsi,
                 const double *sc, const int64_t *irn, double factor, int32_t cap,
                 double *x_out, double *dpsi_out, double *sc_out, int64_t *irn_out);
 
-/* Single-particle-move DMC sweep with Sherman-Morrison inverse updates (the north-star
- * algorithm; no reference counterpart).  One step of one walker: particles moved in order
+/* Single-particle-move DMC sweep with Sherman-Morrison inverse updates (the algorithm of
+ * the device's single-particle path; no reference counterpart).  One step of one walker: particles moved in order
  * 0..npart-1; the inverse is rebuilt from scratch every krefresh accepted-or-rejected moves
  * and at the end of the sweep.  st[w] state as for the population arrays.  Outputs the
  * new state, the uncapped weight and the number of accepted moves. */
