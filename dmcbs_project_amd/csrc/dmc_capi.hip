@@ -1004,6 +1004,10 @@ struct dmc_batch {
     bool dirty = true, host_current = true;
     std::vector<char *> captured_a;  // A.x of every system when the graphs were captured
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // one stream per system (default priority) for steps whose phases are all per-system
+    // launches: forked from st after the step's first launch and joined back at its end
+    hipStream_t side[kMaxBatch] = {};
+    hipEvent_t fork = nullptr, join[kMaxBatch] = {};
     static constexpr int kBatchCounters = 64 * kMaxBatch;
 };
 
@@ -1048,10 +1052,44 @@ int batch_phase(dmc_batch *b, BatchPhase ph, int chunk, int opcomp, int *counter
     return hipSuccess;
 }
 
+// the launches of one system's step on stream s when every phase is a per-system launch:
+// the moves of each chunk (with the FP64 refresh between chunks), the end of the sweep, the
+// estimator sums and the device branch
+int batch_system_chain(dmc_batch *b, int i, int opcomp, hipStream_t s) {
+    dmc_ctx *c = b->ctx[b->order[i]];
+    const BatchSys &q = b->host.sys[i];
+    for (int ch = 0; ch < b->nchunk; ++ch) {
+        const int m0 = ch * q.k;
+        if (m0 >= q.npart) break;
+        const int m1 = m0 + q.k < q.npart ? m0 + q.k : q.npart;
+        if (ch > 0)
+            LAUNCHCHK(launch_sm_refresh(q.nup, c->ds, c->B.x, c->A.sinv, c->T, c->capacity, s));
+        LAUNCHCHK(launch_sm_moves(q.nup, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
+                                  m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1, c->T,
+                                  c->capacity, s));
+    }
+    LAUNCHCHK(launch_sm_end(q.nup, c->ds, c->A, c->B, c->T, opcomp, c->capacity, s));
+    LAUNCHCHK(launch_est_reduce(c->T, -1, c->est_acc, s));
+    LAUNCHCHK(launch_branch_device(c->ds, c->A, c->B, c->T, c->capacity, s));
+    return DMC_OK;
+}
+
 // the launches of one batch step on b->st (captured, or run directly by the tuner)
 int batch_enqueue(dmc_batch *b, int opcomp) {
     const int ns = int(b->order.size());
     LAUNCHCHK(launch_batch_begin(b->args, b->counters, dmc_batch::kBatchCounters, b->st));
+    if (b->wpg[0] == 0 && b->wpg[2] == 0 && (b->nchunk == 1 || b->wpg[1] == 0)) {
+        // no fused launch: the systems share nothing, so their chains run side by side
+        HIPCHK(hipEventRecord(b->fork, b->st));
+        for (int i = 0; i < ns; ++i) {
+            HIPCHK(hipStreamWaitEvent(b->side[i], b->fork, 0));
+            int e = batch_system_chain(b, i, opcomp, b->side[i]);
+            if (e) return e;
+            HIPCHK(hipEventRecord(b->join[i], b->side[i]));
+            HIPCHK(hipStreamWaitEvent(b->st, b->join[i], 0));
+        }
+        return DMC_OK;
+    }
     int slot = 0;
     for (int c = 0; c < b->nchunk; ++c) {
         if (c > 0)
@@ -1164,10 +1202,17 @@ int dmc_batch_create(dmc_ctx *const *ctxs, int32_t n, dmc_batch **out) {
         hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipStreamCreate(&b->st) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&b->counters), sizeof(int) * dmc_batch::kBatchCounters) != hipSuccess ||
-        hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+        hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         dmc_batch_destroy(b);
         return fail(DMC_ERR_HIP, "batch set-up failed");
     }
+    for (size_t i = 0; i < b->ctx.size(); ++i)
+        if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess) {
+            dmc_batch_destroy(b);
+            return fail(DMC_ERR_HIP, "batch set-up failed");
+        }
     for (dmc_ctx *c : b->ctx) (void)hipStreamSynchronize(c->st);
     int e = batch_upload_populations(b);
     if (e) {
@@ -1186,6 +1231,14 @@ int dmc_batch_destroy(dmc_batch *b) {
     if (b->counters) (void)hipFree(b->counters);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
+    for (int i = 0; i < kMaxBatch; ++i) {
+        if (b->side[i]) {
+            (void)hipStreamSynchronize(b->side[i]);
+            (void)hipStreamDestroy(b->side[i]);
+        }
+        if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
     return DMC_OK;
