@@ -209,7 +209,9 @@ int dmc_batch_run(dmc_batch *b, int32_t nsteps);  /* opcomp on the last step */
 int dmc_batch_sync(dmc_batch *b, int32_t *nt);
 /* Launch setting of the three phases: walkers per workgroup (1, 2, 4 or 8; one walker per
  * 64-lane wave) of the one persistent launch over all systems, or 0 for one launch of the
- * single-system kernel per system (all inside the step graph). */
+ * single-system kernel per system (all inside the step graph; when no phase uses the
+ * persistent launch, each system's launches run on a stream of their own inside the graph,
+ * side by side). */
 int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end);
 /* Autotuner: times reps launches of every phase at the settings 1, 2, 4, 8 and 0 on the
  * resident populations (state left unchanged) and keeps the fastest.  wpg_out[3] = the
