@@ -285,7 +285,8 @@ int run_obs(dmc_ctx *c, int ntab, int ntabq, std::vector<double> &sums) {
     const int nsum = 1 + ng + 6 * nq;
     sums.assign(nsum, 0.0);
     if (nw == 0) return DMC_OK;
-    if (obs_lds_bytes(np, ntab, nq, mmax) > kObsLdsMax)
+    const bool part_global = obs_lds_bytes(np, ntab, nq, mmax, false) > kObsLdsMax;
+    if (obs_lds_bytes(np, ntab, nq, mmax, true) > kObsLdsMax)
         return fail(DMC_ERR_ARG, "too many bins or q vectors for the observables kernel");
     const int nblk = obs_blocks(nw), nseg = (nblk + 31) / 32;
     const size_t nd = (size_t(nblk + nseg + 1) * nsum + 1) & ~size_t(1);
@@ -315,6 +316,7 @@ int run_obs(dmc_ctx *c, int ntab, int ntabq, std::vector<double> &sums) {
     a.twopi_l = 2.0 * (4.0 * std::atan(1.0)) / c->sys.el;
     a.nsum = nsum;
     a.nblk = nblk;
+    a.part_global = part_global ? 1 : 0;
     a.part = part;
     LAUNCHCHK(launch_obs(a, tmp, out, c->st));
     HIPCHK(hipMemcpyAsync(sums.data(), out, sizeof(double) * nsum, hipMemcpyDeviceToHost, c->st));

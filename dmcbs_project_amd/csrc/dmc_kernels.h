@@ -69,10 +69,11 @@ struct ObsArgs {
     int nq, mmax;            // 0: no S(q); largest |component| of the vectors
     double twopi_l;
     int nsum, nblk;
+    int part_global;  // partial sums in part (each block its row) instead of LDS
     double *part;
 };
 constexpr int kObsBlocks = 1024;
-size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax);
+size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax, bool part_global);
 int obs_blocks(int nw);
 // k_obs into a.part, then a fixed-order reduction over the blocks: tmp holds
 // ceil(nblk/32) rows of nsum, out one row

@@ -27,8 +27,10 @@ __global__ __launch_bounds__(kObsThreads) void k_obs(ObsArgs a) {
     const int t = threadIdx.x;
     const int np = a.npart, nup = a.nup, ntab = a.ntab, nq = a.nq, mp = a.mmax + 1;
     const int ng = ntab > 0 ? 3 * (ntab + 1) + 6 * (2 * ntab + 1) : 0;
-    double *part = sm;                                          // [nsum]
-    double *xs = part + ((a.nsum + 1) & ~1);                    // [np][3]
+    // partial sums [nsum] in LDS, or (tables too large for it) in the block's own row of the
+    // output, read and written by the same owner threads
+    double *part = a.part_global ? a.part + size_t(blockIdx.x) * a.nsum : sm;
+    double *xs = a.part_global ? sm : part + ((a.nsum + 1) & ~1);  // [np][3]
     double2 *pw = reinterpret_cast<double2 *>(xs + ((3 * np + 1) & ~1));  // [np][3][mp]
     int *cnt = reinterpret_cast<int *>(pw + (nq > 0 ? np * 3 * mp : 0));  // [ng]
 
@@ -161,8 +163,10 @@ __global__ __launch_bounds__(kObsThreads) void k_obs(ObsArgs a) {
         if (t == 0) part[0] += wt;
         __syncthreads();
     }
-    double *out = a.part + size_t(b) * a.nsum;
-    for (int e = t; e < a.nsum; e += kObsThreads) out[e] = part[e];
+    if (!a.part_global) {
+        double *out = a.part + size_t(b) * a.nsum;
+        for (int e = t; e < a.nsum; e += kObsThreads) out[e] = part[e];
+    }
 }
 
 // out[s][e] = sum over rows r in segment s of in[r][e], rows in order
@@ -180,10 +184,10 @@ __global__ __launch_bounds__(kObsThreads) void k_obs_reduce(const double *in, in
 
 }  // namespace
 
-size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax) {
+size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax, bool part_global) {
     const int ng = ntab > 0 ? 3 * (ntab + 1) + 6 * (2 * ntab + 1) : 0;
     const int nsum = 1 + ng + 6 * nq;
-    size_t d = size_t((nsum + 1) & ~1) + size_t((3 * npart + 1) & ~1);
+    size_t d = (part_global ? 0 : size_t((nsum + 1) & ~1)) + size_t((3 * npart + 1) & ~1);
     d += 2 * size_t(nq > 0 ? npart * 3 * (mmax + 1) : 0);
     return d * sizeof(double) + sizeof(int) * size_t(ng);
 }
@@ -191,7 +195,7 @@ size_t obs_lds_bytes(int npart, int ntab, int nq, int mmax) {
 int obs_blocks(int nw) { return nw < kObsBlocks ? nw : kObsBlocks; }
 
 int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st) {
-    const size_t lds = obs_lds_bytes(a.npart, a.ntab, a.nq, a.mmax);
+    const size_t lds = obs_lds_bytes(a.npart, a.ntab, a.nq, a.mmax, a.part_global != 0);
     hipLaunchKernelGGL(k_obs, dim3(a.nblk), dim3(kObsThreads), lds, st, a);
     // block partials [nblk][nsum] -> segments of 32 rows -> one row
     return launch_rows_reduce(a.part, a.nblk, a.nsum, 32, tmp, out, st);
