@@ -2,7 +2,9 @@
 
 The device library libdmc_gpu.so (HIP kernels for gfx950 behind the C-ABI of
 include/dmc_gpu.h) does the work; this package is a thin host-side binding whose classes
-mirror the reference's Fortran module interfaces (step, stack, wavefunction).
+mirror the reference's Fortran module interfaces (step, stack, wavefunction).  The modules
+``observables`` (g(r), density profile, S(q), n(k): gofr, sofq, pdistribution) and ``deck``
+(the positional input deck and the walker file fort.9) complete the host side.
 """
 from .lib import (DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC, DMCError,
                   load)
