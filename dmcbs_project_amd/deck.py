@@ -13,6 +13,7 @@
 """
 import numpy as np
 
+from . import lib as L
 from .api import Params
 from .observables import fortran_e
 
@@ -64,7 +65,8 @@ class _Reader:
 
 def read_deck(text, **device):
     """The reference's deck -> (Params, run settings).  device: Params fields the deck does
-    not hold (precision, movetype, krefresh, capacity)."""
+    not hold (precision, movetype, krefresh, capacity); variational decks (idmc = 0, -1) run
+    the FP64 Metropolis step."""
     r = _Reader(text)
     run = {}
     run["seed"] = r.int()
@@ -130,6 +132,11 @@ def read_deck(text, **device):
         run["optnodes"] = r.logical()
         run["delta"] = r.real()
         run["eps"] = r.real()
+    if run["idmc"] in (0, -1):
+        # variational runs (step1 with idmc = 0, step.f90:78-122), as the Fortran host maps them
+        device = dict(device, movetype=L.DMC_VMC, precision=L.DMC_FP64)
+    elif run["idmc"] != 1:
+        raise ValueError("the engine runs DMC (idmc = 1) and VMC (idmc = 0, -1)")
     p = Params(npart=npart, norb=norb, ntab=ntab, fn=fn, units=units, nwalk=nwalk, rho=rho,
                hbar=hbar, mass=mass[0], dt=dt, etrial=etrial, dcut=dcut, wtcut=wtcut,
                popup=popup, popdn=popdn, bbeta=bbeta, dbeta=dbeta, bfact=bfact, v0=v0,
