@@ -22,6 +22,9 @@ namespace {
 
 constexpr int kObsThreads = 256;
 
+// PG: partial sums in the block's row of the output instead of LDS (a template parameter,
+// so that each variant addresses its partial sums in one known memory space)
+template <bool PG>
 __global__ __launch_bounds__(kObsThreads) void k_obs(ObsArgs a) {
     extern __shared__ double sm[];
     const int t = threadIdx.x;
@@ -29,8 +32,8 @@ __global__ __launch_bounds__(kObsThreads) void k_obs(ObsArgs a) {
     const int ng = ntab > 0 ? 3 * (ntab + 1) + 6 * (2 * ntab + 1) : 0;
     // partial sums [nsum] in LDS, or (tables too large for it) in the block's own row of the
     // output, read and written by the same owner threads
-    double *part = a.part_global ? a.part + size_t(blockIdx.x) * a.nsum : sm;
-    double *xs = a.part_global ? sm : part + ((a.nsum + 1) & ~1);  // [np][3]
+    double *part = PG ? a.part + size_t(blockIdx.x) * a.nsum : sm;
+    double *xs = PG ? sm : part + ((a.nsum + 1) & ~1);  // [np][3]
     double2 *pw = reinterpret_cast<double2 *>(xs + ((3 * np + 1) & ~1));  // [np][3][mp]
     int *cnt = reinterpret_cast<int *>(pw + (nq > 0 ? np * 3 * mp : 0));  // [ng]
 
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(kObsThreads) void k_obs(ObsArgs a) {
         if (t == 0) part[0] += wt;
         __syncthreads();
     }
-    if (!a.part_global) {
+    if constexpr (!PG) {
         double *out = a.part + size_t(b) * a.nsum;
         for (int e = t; e < a.nsum; e += kObsThreads) out[e] = part[e];
     }
@@ -196,7 +199,10 @@ int obs_blocks(int nw) { return nw < kObsBlocks ? nw : kObsBlocks; }
 
 int launch_obs(const ObsArgs &a, double *tmp, double *out, hipStream_t st) {
     const size_t lds = obs_lds_bytes(a.npart, a.ntab, a.nq, a.mmax, a.part_global != 0);
-    hipLaunchKernelGGL(k_obs, dim3(a.nblk), dim3(kObsThreads), lds, st, a);
+    if (a.part_global)
+        hipLaunchKernelGGL(k_obs<true>, dim3(a.nblk), dim3(kObsThreads), lds, st, a);
+    else
+        hipLaunchKernelGGL(k_obs<false>, dim3(a.nblk), dim3(kObsThreads), lds, st, a);
     // block partials [nblk][nsum] -> segments of 32 rows -> one row
     return launch_rows_reduce(a.part, a.nblk, a.nsum, 32, tmp, out, st);
 }
