@@ -32,6 +32,16 @@ def build():
 
 
 _lib = None
+_planted = {}
+
+
+def planted(k):
+    """The oracle built with planted mistake k (DMCO_PLANT in dmc_oracle.c), for the tests
+    that show a check catches that mistake"""
+    if k not in _planted:
+        subprocess.run(["make", "-s", "-C", HERE, f"_build/libdmc_oracle_plant{k}.so"], check=True)
+        _planted[k] = _bind(C.CDLL(os.path.join(HERE, "_build", f"libdmc_oracle_plant{k}.so")))
+    return _planted[k]
 
 
 def lib():
@@ -39,61 +49,68 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.dmco_create.restype = C.c_void_p
-        L.dmco_create.argtypes = [C.POINTER(Params)]
-        L.dmco_destroy.argtypes = [C.c_void_p]
-        L.dmco_info.argtypes = [C.c_void_p, _f64p]
-        L.dmco_kvec.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
-        L.dmco_kvec.restype = C.c_int
-        L.dmco_tables.argtypes = [C.c_void_p] + [_f64p] * 7
-        for f in ("dmco_radial", "dmco_beta"):
-            getattr(L, f).argtypes = [C.c_void_p, C.c_double, _f64p]
-        L.dmco_derbeta.argtypes = [C.c_void_p, C.c_double, _f64p]
-        L.dmco_vpot.argtypes = [C.c_void_p, C.c_double]
-        L.dmco_vpot.restype = C.c_double
-        L.dmco_getphi.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p]
-        L.dmco_bcs.argtypes = [C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p, _f64p]
-        L.dmco_hpsi.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
-        L.dmco_ratio1.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
-        L.dmco_ratio1.restype = C.c_double
-        L.dmco_getder.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
-        L.dmco_set_params.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double]
-        L.dmco_ran1.argtypes = [C.POINTER(C.c_int64)]
-        L.dmco_ran1.restype = C.c_double
-        L.dmco_ran2.argtypes = [C.POINTER(C.c_int64)]
-        L.dmco_ran2.restype = C.c_double
-        L.dmco_gaussian.argtypes = [C.POINTER(C.c_int64), C.c_int32, _f64p]
-        L.dmco_init_walkers.argtypes = [C.c_void_p, C.c_int64, C.c_int32, _f64p, _i64p]
-        L.dmco_init_hpsi.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
-        L.dmco_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                 C.c_int32, _f64p, _f64p, _f64p, _f64p, _i64p, _f64p,
-                                 C.c_int32]
-        L.dmco_vmc_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                     _f64p, _f64p, _f64p, _f64p, _i64p, C.c_int32]
-        L.dmco_branch.argtypes = [C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                  C.c_double, C.c_int32, _f64p, _f64p, _f64p, _i64p]
-        L.dmco_branch.restype = C.c_int
-        L.dmco_dmc_step.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                    C.c_int32, _f64p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                    _i32p, C.c_int32]
-        L.dmco_dmc_step.restype = C.c_int
-        L.dmco_sm_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
-                                    C.c_int32, _f64p, _f64p, _f64p, _i64p, _f64p, _i32p,
-                                    C.c_int32]
-        L.dmco_kshells.argtypes = [C.c_double, C.c_int32, _i32p, _f64p, C.c_int32]
-        L.dmco_kshells.restype = C.c_int
-        L.dmco_addgofr.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, _f64p,
-                                   C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p]
-        L.dmco_addsofq.argtypes = [C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
-                                   _f64p, _f64p, _f64p]
-        L.dmco_enk_shells.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i32p,
-                                      C.POINTER(C.c_int32)]
-        L.dmco_enk_shells.restype = C.c_int
-        L.dmco_addenk.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _i32p, _f64p, C.c_int64,
-                                  C.c_double, _f64p, _f64p, _f64p]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+def _bind(L):
+    """argument and result types of the oracle's functions"""
+    L.dmco_create.restype = C.c_void_p
+    L.dmco_create.argtypes = [C.POINTER(Params)]
+    L.dmco_destroy.argtypes = [C.c_void_p]
+    L.dmco_info.argtypes = [C.c_void_p, _f64p]
+    L.dmco_kvec.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
+    L.dmco_kvec.restype = C.c_int
+    L.dmco_tables.argtypes = [C.c_void_p] + [_f64p] * 7
+    for f in ("dmco_radial", "dmco_beta"):
+        getattr(L, f).argtypes = [C.c_void_p, C.c_double, _f64p]
+    L.dmco_derbeta.argtypes = [C.c_void_p, C.c_double, _f64p]
+    L.dmco_vpot.argtypes = [C.c_void_p, C.c_double]
+    L.dmco_vpot.restype = C.c_double
+    L.dmco_getphi.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.dmco_bcs.argtypes = [C.c_void_p, _f64p, _f64p, _i32p, _f64p, _f64p, _f64p]
+    L.dmco_hpsi.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+    L.dmco_ratio1.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
+    L.dmco_ratio1.restype = C.c_double
+    L.dmco_getder.argtypes = [C.c_void_p, _f64p, C.c_int32, _f64p]
+    L.dmco_set_params.argtypes = [C.c_void_p, _f64p, C.c_double, C.c_double]
+    L.dmco_ran1.argtypes = [C.POINTER(C.c_int64)]
+    L.dmco_ran1.restype = C.c_double
+    L.dmco_ran2.argtypes = [C.POINTER(C.c_int64)]
+    L.dmco_ran2.restype = C.c_double
+    L.dmco_gaussian.argtypes = [C.POINTER(C.c_int64), C.c_int32, _f64p]
+    L.dmco_init_walkers.argtypes = [C.c_void_p, C.c_int64, C.c_int32, _f64p, _i64p]
+    L.dmco_init_hpsi.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
+    L.dmco_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                             C.c_int32, _f64p, _f64p, _f64p, _f64p, _i64p, _f64p,
+                             C.c_int32]
+    L.dmco_vmc_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                 _f64p, _f64p, _f64p, _f64p, _i64p, C.c_int32]
+    L.dmco_branch.argtypes = [C.c_int32, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                              C.c_double, C.c_int32, _f64p, _f64p, _f64p, _i64p]
+    L.dmco_branch.restype = C.c_int
+    L.dmco_dmc_step.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                C.c_int32, _f64p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                _i32p, C.c_int32]
+    L.dmco_dmc_step.restype = C.c_int
+    L.dmco_sm_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                C.c_int32, _f64p, _f64p, _f64p, _i64p, _f64p, _i32p,
+                                C.c_int32]
+    L.dmco_sm_step1_trace.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
+                                      C.c_int32, _f64p, _f64p, _f64p, _i64p, _f64p, _i32p,
+                                      _f64p, C.c_int32]
+    L.dmco_kshells.argtypes = [C.c_double, C.c_int32, _i32p, _f64p, C.c_int32]
+    L.dmco_kshells.restype = C.c_int
+    L.dmco_addgofr.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, _f64p,
+                               C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.dmco_addsofq.argtypes = [C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
+                               _f64p, _f64p, _f64p]
+    L.dmco_enk_shells.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i32p,
+                                  C.POINTER(C.c_int32)]
+    L.dmco_enk_shells.restype = C.c_int
+    L.dmco_addenk.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _i32p, _f64p, C.c_int64,
+                              C.c_double, _f64p, _f64p, _f64p]
+    return L
 
 
 def enk_shells(ntab, maxk=8192):
@@ -164,7 +181,7 @@ VORB10 = [0.00197519728006648144, 0.00249764752028349595, 0.00194383232328550247
 class System:
     """One system (deck) of the oracle."""
 
-    def __init__(self, vorb=None, **kw):
+    def __init__(self, vorb=None, lib_=None, **kw):
         p = dict(DEFAULTS)
         p.update(kw)
         self.params = p
@@ -175,7 +192,7 @@ class System:
         for i, v in enumerate(vo):
             P.vorb[i] = v
         self.vorb = np.array(vo, dtype=np.float64)
-        self.L = lib()
+        self.L = lib_ if lib_ is not None else lib()
         self.h = self.L.dmco_create(C.byref(P))
         if not self.h:
             raise RuntimeError("dmco_create failed")
@@ -339,6 +356,23 @@ class System:
                              int(krefresh), xo.reshape(-1), do.reshape(-1), so.reshape(-1), ro,
                              wt, nacc, nthreads)
         return xo, do, so, ro, wt, nacc
+
+    def sm_step1_trace(self, x, dpsi, sc, irn, krefresh, nthreads=0):
+        """sm_step1 plus the record of every move, [nw][npart][TRACE] (dmc_oracle.h)"""
+        nw = len(x)
+        xo, do = np.zeros_like(x), np.zeros_like(dpsi)
+        so, ro, wt = np.zeros_like(sc), np.zeros_like(irn), np.zeros(nw)
+        nacc = np.zeros(nw, np.int32)
+        tr = np.zeros(nw * self.npart * SM_TRACE)
+        self.L.dmco_sm_step1_trace(self.h, nw, np.ascontiguousarray(x).reshape(-1),
+                                   np.ascontiguousarray(dpsi).reshape(-1),
+                                   np.ascontiguousarray(sc).reshape(-1),
+                                   np.ascontiguousarray(irn), int(krefresh), xo.reshape(-1),
+                                   do.reshape(-1), so.reshape(-1), ro, wt, nacc, tr, nthreads)
+        return xo, do, so, ro, wt, nacc, tr.reshape(nw, self.npart, SM_TRACE)
+
+
+SM_TRACE = 24  # DMCO_SM_TRACE
 
 
 def calder_updatepar(system, configs, delta, eps, optbeta=False):

@@ -24,6 +24,14 @@
 
 #define MAXK 512
 
+/* Planted mistakes for the tests that show the single-particle checks catch them (built
+ * only by those tests, into a separate library): 1 = sign of the down-spin determinant
+ * gradient, 2 = factor of the Green's-function term, 3 = a dropped factor 2 on the Jastrow
+ * part of the acceptance.  0 (the default) is the oracle. */
+#ifndef DMCO_PLANT
+#define DMCO_PLANT 0
+#endif
+
 struct dmco_ctx {
     dmco_params p;
     int npart, n;               /* n = particles per spin */
@@ -1085,7 +1093,11 @@ static void det_grad(const dmco_ctx *c, const double *x, const double *sinv, int
         int jd = ip - n;
         for (int i = 0; i < n; i++) {
             pairphi(c, x + 3 * i, xp, &f, gg, &l);
+#if DMCO_PLANT == 1
+            for (int d = 0; d < 3; d++) g[d] += sinv[jd + n * i] * gg[d];
+#else
             for (int d = 0; d < 3; d++) g[d] -= sinv[jd + n * i] * gg[d];
+#endif
         }
     }
     for (int d = 0; d < 3; d++) g[d] *= scale;
@@ -1141,7 +1153,7 @@ static void sm_update(const dmco_ctx *c, const double *x, double *sinv, int ip,
 
 static void sm_one(const dmco_ctx *c, const double *x, const double *dpsi, const double *sc,
                    int64_t irn, int kref, double *xo, double *dpsio, double *sco,
-                   int64_t *irno, double *wto, int32_t *nacc) {
+                   int64_t *irno, double *wto, int32_t *nacc, double *tr) {
     int np = c->npart, n = c->n;
     double el = c->el, hbar = c->p.hbar, sig = c->sigma, sig2 = sig * sig, sm = sqrt(c->p.mass);
     double *sinv = malloc(sizeof(double) * (n * n + 3 * np)), *dphi = sinv + n * n;
@@ -1155,6 +1167,7 @@ static void sm_one(const dmco_ctx *c, const double *x, const double *dpsi, const
         if (ip > 0 && ip % kref == 0) dmco_bcs(c, xo, &phil, &is, dphi, &d2phi, sinv);
         double *xi = xo + 3 * ip;
         double gd[3], gj[3], go[3], gn[3], g3[3], xn[3], dx[3], dro[3], drn[3];
+        double *t = tr ? tr + (size_t)DMCO_SM_TRACE * ip : NULL;
         det_grad(c, xo, sinv, ip, xi, 1.0, gd);
         jas_grad(c, xo, ip, xi, gj);
         for (int d = 0; d < 3; d++) go[d] = gd[d] + gj[d];
@@ -1170,18 +1183,48 @@ static void sm_one(const dmco_ctx *c, const double *x, const double *dpsi, const
         double rho_d = det_ratio(c, xo, sinv, ip, xn);
         double rho_j = jastrow_ratio(c, xo, ip, xn);
         double u = dmco_ran1(&s);
+        if (t) {
+            t[0] = ip;
+            for (int d = 0; d < 3; d++) {
+                t[1 + d] = xi[d];
+                t[4 + d] = go[d];
+                t[7 + d] = dro[d];
+                t[10 + d] = dx[d];
+                t[13 + d] = xn[d];
+                t[21 + d] = NAN;
+            }
+            t[16] = rho_d;
+            t[17] = rho_j;
+            t[18] = u;
+            t[19] = NAN;
+            t[20] = 0.0;
+        }
         if (c->p.fn && !(rho_d > 0.0)) continue; /* node crossing: reject */
         det_grad(c, xo, sinv, ip, xn, 1.0 / rho_d, gd);
         jas_grad(c, xo, ip, xn, gj);
+#if DMCO_PLANT == 3
+        double logp = 2.0 * log(fabs(rho_d)) + log(rho_j);
+#else
         double logp = 2.0 * (log(fabs(rho_d)) + log(rho_j));
+#endif
         for (int d = 0; d < 3; d++) {
             gn[d] = gd[d] + gj[d];
             drn[d] = gn[d] * sig2;
             if (c->p.dcut >= 0.0) drn[d] = fmin(sig * c->p.dcut, fmax(-sig * c->p.dcut, drn[d]));
+#if DMCO_PLANT == 2
+            logp += (dro[d] + drn[d]) * (dro[d] - drn[d] - 2.0 * dx[d]) / sig2;
+#else
             logp += (dro[d] + drn[d]) * (dro[d] - drn[d] - 2.0 * dx[d]) / (2.0 * sig2);
+#endif
         }
         double prob = exp(fmin(0.0, logp));
-        if (u < prob) {
+        const int ok = u < prob;
+        if (t) {
+            t[19] = logp;
+            t[20] = ok;
+            for (int d = 0; d < 3; d++) t[21 + d] = gn[d];
+        }
+        if (ok) {
             sm_update(c, xo, sinv, ip, xn, rho_d);
             for (int d = 0; d < 3; d++) xi[d] = xn[d];
             acc++;
@@ -1205,6 +1248,14 @@ void dmco_sm_step1(const dmco_ctx *c, int32_t nw, const double *x, const double 
                    const double *sc, const int64_t *irn, int32_t krefresh, double *x_out,
                    double *dpsi_out, double *sc_out, int64_t *irn_out, double *wt_out,
                    int32_t *nacc_out, int32_t nthreads) {
+    dmco_sm_step1_trace(c, nw, x, dpsi, sc, irn, krefresh, x_out, dpsi_out, sc_out, irn_out,
+                        wt_out, nacc_out, NULL, nthreads);
+}
+
+void dmco_sm_step1_trace(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                         const double *sc, const int64_t *irn, int32_t krefresh, double *x_out,
+                         double *dpsi_out, double *sc_out, int64_t *irn_out, double *wt_out,
+                         int32_t *nacc_out, double *trace, int32_t nthreads) {
     int np = c->npart;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1213,7 +1264,8 @@ void dmco_sm_step1(const dmco_ctx *c, int32_t nw, const double *x, const double 
     for (int w = 0; w < nw; w++) {
         size_t o = (size_t)3 * np * w;
         sm_one(c, x + o, dpsi + o, sc + 6 * w, irn[w], krefresh, x_out + o, dpsi_out + o,
-               sc_out + 6 * w, irn_out + w, wt_out + w, nacc_out + w);
+               sc_out + 6 * w, irn_out + w, wt_out + w, nacc_out + w,
+               trace ? trace + (size_t)DMCO_SM_TRACE * np * w : NULL);
     }
 }
 

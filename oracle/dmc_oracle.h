@@ -101,6 +101,18 @@ void dmco_sm_step1(const dmco_ctx *c, int32_t nw, const double *x, const double 
                    const double *sc, const int64_t *irn, int32_t krefresh, double *x_out,
                    double *dpsi_out, double *sc_out, int64_t *irn_out, double *wt_out,
                    int32_t *nacc_out, int32_t nthreads);
+/* The same sweep with a record of every move: trace[w][ip][DMCO_SM_TRACE] =
+ *   0 ip, 1-3 the mover's position before the move, 4-6 grad_ip ln psi there (determinant
+ *   through the carried inverse + Jastrow), 7-9 the drift after the cut, 10-12 the
+ *   displacement (drift + sigma * gaussian), 13-15 the proposed position (wrapped),
+ *   16 the determinant ratio, 17 the Jastrow ratio, 18 the acceptance draw, 19 the log of
+ *   the Metropolis ratio (NaN for a move refused by the node test), 20 accepted (0/1),
+ *   21-23 grad_ip ln psi at the proposed position (NaN for a refused move). */
+#define DMCO_SM_TRACE 24
+void dmco_sm_step1_trace(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                         const double *sc, const int64_t *irn, int32_t krefresh, double *x_out,
+                         double *dpsi_out, double *sc_out, int64_t *irn_out, double *wt_out,
+                         int32_t *nacc_out, double *trace, int32_t nthreads);
 
 /* Observables (the operators of operators.f90:92-108) on one configuration x[npart][3]. */
 /* setupk with ndim = 3 (kshell.f90:6-156) for nsh0 shells of box side el: numsh[nsh0],

@@ -217,6 +217,63 @@ def main_vmc():
                 dt=0.002, seed=11)
     opt_fixture(14, 3, random_configs(14, 64, 6014), "N14")
     opt_fixture(66, 5, random_configs(66, 16, 6066), "N66")
+    vmc_long_fixture()
+
+
+def _lcg_jump(s, k):
+    """ran1's state after k draws from state s (ran.f90:8-27: s <- s * A + C mod 2^48)"""
+    for _ in range(k):
+        s = (s * 44485709377909 + 96309754297) % (1 << 48)
+    return s
+
+
+def vmc_long_fixture(n=2, nwalk=512, dt=0.05, dcut=1.0, first=500, last=3000, every=50,
+                     seed=11):
+    """A long variational run (step1 with idmc = 0: all-particle Metropolis moves that sample
+    |psi|^2 exactly), N = 2 (psi = phi(r_1 - r_2) J(r_12): one relative coordinate, fast
+    mixing).  Kept: ln|psi|, the local energy and the minimum-image pair distance of every
+    walker every 50 steps from step 500 to 3000 (51 samples of 512 independent chains, in
+    the walker order of step 500), against which a single-particle Metropolis chain of the
+    same |psi|^2 is compared.  The drift is cut at sigma (dcut = 1): without the cut the
+    drift sigma^2/r near the pair's coalescence overshoots, the chain rarely enters or
+    leaves that region, and 3000 steps at this dt do not converge there (ln|psi| 5 sigma
+    below a direct quadrature of |psi|^2; at dt = 0.01 or with the cut they agree)."""
+    deckargs = dict(idmc=0, npart=n, norb=3, nwalk=nwalk, nav=last // 20, nstep=20, dt=dt,
+                    dcut=dcut, seed=seed)
+    deck, p = refdeck.make_deck(**deckargs)
+    el_box = box(n)
+    dumps = np.arange(first, last + 1, every, dtype=np.int64)
+    data, _ = refrun.run("dmc", deck, pack([len(dumps)], dumps))
+    r = Reader(data)
+    nsteps = r.i()
+    psil, el, r12, key = [], [], [], None
+    # a variational step draws 3N gaussians (3N ran1 draws, N even) and one acceptance draw
+    jump = every * (3 * n + 1)
+    for jj in range(1, nsteps + 1):
+        r.i(2)
+        if jj in dumps:
+            irn, sc, x, _ = read_dump(r, n)
+            if key is None:
+                order = np.arange(len(irn))
+                key = [int(v) for v in irn]
+            else:
+                # the stack's pop order changes from step to step; the walker of slot i is
+                # the one whose RNG state is slot i's state jumped ahead by the draws between
+                key = [_lcg_jump(v, jump) for v in key]
+                pos = {int(v): k for k, v in enumerate(irn)}
+                order = np.array([pos[v] for v in key])
+            d = x[order, 0] - x[order, 1]
+            d -= el_box * np.rint(d / el_box)
+            psil.append(sc[order, 1])
+            el.append(-p["hbar"] * sc[order, 2] + sc[order, 3])
+            r12.append(np.sqrt(np.sum(d * d, axis=1)))
+        if jj % p["nstep"] == 0:
+            r.f(28)
+    assert r.done()
+    np.savez_compressed(os.path.join(HERE, f"vmclong_N{n}.npz"), npart=n, norb=p["norb"],
+                        nwalk=p["nwalk"], dt=p["dt"], dcut=p["dcut"], seed=p["seed"],
+                        steps=dumps, psil=np.array(psil, np.float32),
+                        elocal=np.array(el, np.float32), r12=np.array(r12, np.float32), deck=deck)
 
 
 def _table(text):
@@ -314,6 +371,8 @@ def main():
 if __name__ == "__main__":
     if "--vmc" in sys.argv:
         main_vmc()
+    elif "--vmclong" in sys.argv:
+        vmc_long_fixture()
     elif "--obs" in sys.argv:
         main_obs()
     elif "--enk" in sys.argv:
