@@ -671,6 +671,16 @@ int dmc_branch(dmc_ctx *c, double factor, int32_t *nt) {
             return fail(DMC_ERR_ARG, "the branch that follows dmc_propagate takes factor 1");
         c->pending = false;
         e = compact(c, c->B, c->A, n, total);
+        if (e == DMC_ERR_OVERFLOW && c->ds.keep_sinv) {
+            // the population stays the one before the propagation, but its moves updated
+            // A.sinv in place: rebuild that inverse from A.x (B's arrays, which the failed
+            // step leaves unused, take the other outputs) so that stepping can go on
+            std::string msg = g_err;
+            LAUNCHCHK(launch_eval(c->sys.n, c->ds, c->A.x, n, c->B.psil, c->B.is, c->B.dpsi,
+                                  c->B.d2psi, c->B.v, c->A.sinv, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            return fail(e, msg);
+        }
         if (e) return e;
         for (int i = 0; i < c->nev_pending; ++i) {  // the stream was synchronised by compact()
             float ms = 0.f;

@@ -111,7 +111,11 @@ int dmc_step(dmc_ctx *c, int32_t opcomp, int32_t *nt);
  * dmc_branch replaces branch (step.f90:287-315).  Right after dmc_propagate it takes
  * factor 1 and applies the drawn multiplicities; otherwise it is a population-control pass
  * with multiplicities int(weight*factor + u).  nt = the population afterwards.  DMC_VMC
- * populations are not branched (dmcbcs.f90:441): dmc_propagate leaves them in place. */
+ * populations are not branched (dmcbcs.f90:441): dmc_propagate leaves them in place.
+ * DMC_ERR_OVERFLOW (more copies than the capacity) discards the step: the population is
+ * the one before dmc_propagate, with its inverses rebuilt, and stepping can continue (the
+ * estimator sums of the discarded step stay accumulated; the reference stops there,
+ * stack.f90:88-91). */
 int dmc_propagate(dmc_ctx *c, int32_t opcomp);
 int dmc_branch(dmc_ctx *c, double factor, int32_t *nt);
 /* Run nsteps steps (opcomp on the last), returning the final population. */

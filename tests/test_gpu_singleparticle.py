@@ -63,14 +63,17 @@ def test_fp64_sweep_matches_oracle(n, norb, nw, k, dcut):
         x, dpsi, sc, irn = X, D, SC, R
 
 
-@pytest.mark.parametrize("n,norb,nw", [(14, 3, 128), (66, 5, 32)])
-def test_mixed_sweep_tracks_fp64(n, norb, nw):
+@pytest.mark.parametrize("n,norb,nw,k", [(14, 3, 128, 14), (14, 3, 128, 5), (66, 5, 32, 66),
+                                         (66, 5, 32, 33), (66, 5, 32, 16)])
+def test_mixed_sweep_tracks_fp64(n, norb, nw, k):
+    """the mixed sweep (FP32 moves, FP64 refresh every k moves) against the oracle's FP64
+    sweep with the same k"""
     from dmcbs_project_amd import DMC_MIXED
-    e, s = engines(n, norb, nw, n, DMC_MIXED, dcut=2.0)
+    e, s = engines(n, norb, nw, k, DMC_MIXED, dcut=2.0)
     x, irn = s.init_walkers(9, nw)
     dpsi, sc = s.init_hpsi(x)
     e.push_walkers(x, irn)
-    X, D, SC, R, nacc, _ = oracle_step(s, x, dpsi, sc, irn, n)
+    X, D, SC, R, nacc, _ = oracle_step(s, x, dpsi, sc, irn, k)
     e.step()
     acc, prop = e.acceptance()
     assert abs(acc - int(nacc.sum())) <= max(2, 0.01 * prop)
@@ -140,3 +143,36 @@ def test_headline_configuration_state_is_consistent():
     els = -0.5 * w["d2psi"][sample] + w["v"][sample]
     np.testing.assert_allclose(els, el, rtol=1e-11, atol=1e-10)
     np.testing.assert_allclose(w["dpsi"][sample], h["dpsi"], rtol=1e-10, atol=1e-10)
+
+
+def test_overflow_keeps_a_steppable_population():
+    """A step whose branching exceeds the capacity fails with DMC_ERR_OVERFLOW and leaves the
+    population as it was before the step, inverses included: after a population-control
+    pass the next FP64 sweep still reproduces the oracle's sweep of the same walkers move for
+    move (a stale inverse would change the Sherman-Morrison ratios)."""
+    from dmcbs_project_amd import DMC_FP64, DMCError, DMC_SINGLEPARTICLE, Engine
+    n, norb, nw = 14, 3, 64
+    kw = dict(npart=n, norb=norb, nwalk=nw, dt=0.002, dcut=2.0, popup=1e9, popdn=0.0)
+    e = Engine(capacity=100, precision=DMC_FP64, movetype=DMC_SINGLEPARTICLE, krefresh=n,
+               etrial=1000.0, **kw)
+    e.init_walkers(3)
+    before = e.walkers()
+    with pytest.raises(DMCError) as ei:
+        e.step()
+    assert ei.value.code == 3
+    after = e.walkers()
+    for key in ("x", "irn", "sc", "dpsi"):
+        np.testing.assert_array_equal(after[key], before[key])
+    assert e.branch(0.5) < 100
+    w = e.walkers()
+    import oracle
+    s = oracle.System(etrial=1000.0, **kw)
+    X, D, SC, R, nacc, _ = oracle_step(s, w["x"], w["dpsi"], w["sc"], w["irn"], n)
+    e.acceptance()
+    try:
+        e.step()
+    except DMCError as ex:  # the population may overflow again; the sweep is what counts
+        assert ex.code == 3
+    acc, prop = e.acceptance()
+    assert prop == n * len(w["irn"])
+    assert acc == int(nacc.sum())
