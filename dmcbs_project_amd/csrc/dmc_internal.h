@@ -49,7 +49,7 @@ void enk_shells(int ntab, std::vector<std::array<int, 3>> &ik, std::vector<int> 
                 std::vector<int> &count);
 // Builds everything from the parameters; returns a DMC_* status.
 int build_system(const dmc_params &p, System &s, std::string &err);
-// Rebuilds the beta table for new (b, d) (initbeta, betafun.f90:359-410).
+// Rebuilds the beta table for new (b, d) (initbeta, betafun.f90:74-125).
 void build_beta(System &s, double b, double d);
 // Initial walkers (dmcbcs.f90:237-301): x[nwalk][npart][3], irn[nwalk].
 void initial_walkers(const System &s, int64_t seed, int nwalk, std::vector<double> &x,

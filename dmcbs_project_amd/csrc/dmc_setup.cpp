@@ -143,7 +143,7 @@ inline Dual exp(Dual x) {
 inline double val(double x) { return x; }
 inline double val(const Dual &x) { return x.v; }
 
-// beta0 and beta (betafun.f90:412-456) for the current (b, d, c); T = double or Dual
+// beta0 and beta (betafun.f90:127-171) for the current (b, d, c); T = double or Dual
 template <class T>
 struct Beta {
     T b, d, c, c0, c1, c2, c3, c4;
@@ -186,7 +186,7 @@ double vcosh(const System &s, double v0, double r) {
     return -2.0 * v0 * s.p.amu * s.p.amu * s.tab_hbar / (ch * ch);
 }
 
-// RK4 integration of -hb u'' + (V - E) u = 0 from u(0) = 0, u'(0) = 1 (rungekutta.f90:378)
+// RK4 integration of -hb u'' + (V - E) u = 0 from u(0) = 0, u'(0) = 1 (rungekutta.f90:127)
 int integrate(const System &s, double v0, double e, double hb, double dr, int nt,
               std::vector<double> &u, std::vector<double> &up) {
     u[0] = 0.0;
@@ -207,7 +207,7 @@ int integrate(const System &s, double v0, double e, double hb, double dr, int nt
 }
 
 // Zero-node solution with u'(R)/u(R) = 1/R at R = nt*dr, by bisection on E
-// (rungekutta.f90:259-351); returns f = u/r, f', f'' on [0, nt].
+// (rungekutta.f90:8-100); returns f = u/r, f', f'' on [0, nt].
 int two_body(const System &s, double v0, int nt, double dr, std::vector<double> &f,
              std::vector<double> &fp, std::vector<double> &fpp, std::string &err) {
     const double hb = s.tab_hbar;
@@ -284,7 +284,7 @@ int jastrow_tables(System &s, std::string &err) {
 
 namespace {
 
-// beta table (initbeta, betafun.f90:359-410) for parameters (b, d) of type T
+// beta table (initbeta, betafun.f90:74-125) for parameters (b, d) of type T
 template <class T>
 void beta_table(const System &s, T b, T d, std::vector<T> &bt, std::vector<T> &dbt,
                 std::vector<T> &d2bt) {
@@ -308,7 +308,7 @@ void beta_table(const System &s, T b, T d, std::vector<T> &bt, std::vector<T> &d
            c * (d / 2.0 + d * d / 2.0 + d * d * d / 6.0);
     B.c4 = d / 24.0 + d * d / 12.0 + d * d * d / 12.0 + d * d * d * d / 24.0 +
            d * d * d * d * d / 120.0 - c * (d / 6.0 + d * d / 4.0 + d * d * d / 6.0 + d * d * d * d / 24.0);
-    // number of periodic images needed (betafun.f90:379-384)
+    // number of periodic images needed (betafun.f90:94-99)
     int nj = 0;
     for (double bn = 10.0; std::fabs(bn) > 1.0e-10;) {
         ++nj;

@@ -63,7 +63,7 @@ __device__ __forceinline__ double wrapd(double x, const DevSys &S) {
 }
 
 // Weights of the 4-point Lagrange interpolation on the nodes -1, 0, 1, 2 at t
-// (coshpot.f90:172-179 / betafun.f90:468-475): w0 = -t(t-1)(t-2)/6 and w3 = (t+1)t(t-1)/6
+// (coshpot.f90:172-179 / betafun.f90:183-190): w0 = -t(t-1)(t-2)/6 and w3 = (t+1)t(t-1)/6
 // from p = t(t-1); w2 and w1 from the identities sum w = 1 and sum w x = t that the cubic
 // weights satisfy (10 operations instead of 16)
 template <typename R>
@@ -94,7 +94,7 @@ __device__ __forceinline__ double4 lookup4(const double4 *tab, int idx, double w
                         w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w);
 }
 
-// beta(r) and (beta', lap beta) from the beta table (radialfun, betafun.f90:458-479)
+// beta(r) and (beta', lap beta) from the beta table (radialfun, betafun.f90:173-194)
 __device__ __forceinline__ double beta_val(const DevSys &S, double r) {
     if (S.nobeta || r > S.rangeb) return 0.0;
     double w0, w1, w2, w3;

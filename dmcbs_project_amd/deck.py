@@ -2,7 +2,7 @@
 
 * ``read_deck`` reads the positional stdin deck in the order of the reference's reads:
   the run header (dmcbcs.f90:43-66), setpsi (wavefunction.f90:36-37 and the BCS deck of
-  psislaterbcs.f90:47-68 and betafun.f90:304-306), setphi (coshpot.f90:35-41), the four
+  psislaterbcs.f90:47-68 and betafun.f90:19-21), setphi (coshpot.f90:35-41), the four
   operator table sizes (operators.f90:33-41) and, for optimisation runs, setupopt
   (optimizer.f90:23-27).  It returns the engine's ``Params`` and the run settings; deck
   options the device path does not support raise ``ValueError``, as the Fortran host stops

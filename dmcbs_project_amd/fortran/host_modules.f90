@@ -190,7 +190,7 @@ module wavefunction
    integer(kind=i4), private, save :: npart,norb
 contains
 ! setpsi: nodes, a2 (wavefunction.f90:36-37), then the pairing deck (psislaterbcs.f90:47-68)
-! and the beta deck (betafun.f90:304-306); returns the box side el
+! and the beta deck (betafun.f90:19-21); returns the box side el
    subroutine setpsi(ndim,nbin,el,mass,ntab)
    integer(kind=i4) :: ndim,nbin(:),ntab
    real(kind=r8) :: el,mass(:)

@@ -27,7 +27,7 @@ enum {
     DMC_ERR_HIP = 2,          /* HIP runtime error */
     DMC_ERR_OVERFLOW = 3,     /* population exceeds capacity (stack.f90:88-91) */
     DMC_ERR_SETUP = 4,        /* "Too many states" (kshell.f90:137-140) or RK4 bisection
-                                 failure (rungekutta.f90:336-339) */
+                                 failure (rungekutta.f90:85-88) */
     DMC_ERR_NODEVICE = 5
 };
 
@@ -41,7 +41,7 @@ enum { DMC_FP64 = 0, DMC_MIXED = 1 };
 enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1, DMC_VMC = 2 };
 
 /* Run parameters: the reference's deck (dmcbcs.f90:43-66, psislaterbcs.f90:47-68,
- * betafun.f90:304-306, coshpot.f90:35-41) plus the device options. */
+ * betafun.f90:19-21, coshpot.f90:35-41) plus the device options. */
 typedef struct dmc_params {
     int32_t npart;      /* total particles, nup = ndn = npart/2 */
     int32_t norb;       /* s-wave k-shells */
@@ -61,7 +61,7 @@ typedef struct dmc_params {
 typedef struct dmc_ctx dmc_ctx;
 
 /* Set-up: replaces setpsi/setupbcs (wavefunction.f90:13-96, psislaterbcs.f90:23-153),
- * setbeta (betafun.f90:297-357), setphi (coshpot.f90:14-143) and setstep (step.f90:11-41).
+ * setbeta (betafun.f90:12-72), setphi (coshpot.f90:14-143) and setstep (step.f90:11-41).
  * Builds the k shells, beta and Jastrow/potential tables on the host and uploads them. */
 int dmc_create(const dmc_params *p, dmc_ctx **out);
 int dmc_destroy(dmc_ctx *c);
@@ -76,7 +76,7 @@ int dmc_set_tables(dmc_ctx *c, const double *vt, const double *ut, const double 
                    const double *d2ut, const double *bt, const double *dbt,
                    const double *d2bt);
 /* setdetparam/setbcspar (wavefunction.f90:231-258, psislaterbcs.f90:401-419):
- * alpha_1..alpha_norb, and beta's (b, d) (initbeta, betafun.f90:359-410) */
+ * alpha_1..alpha_norb, and beta's (b, d) (initbeta, betafun.f90:74-125) */
 int dmc_set_params(dmc_ctx *c, const double *vorb, double bbeta, double dbeta);
 
 /* Walker store (stack.f90 create/push, dmcbcs.f90:264-345): load nw walkers and run the
@@ -131,7 +131,7 @@ int dmc_acceptance(dmc_ctx *c, int64_t *accepted, int64_t *proposed);
 
 /* Adjoint: d log psi / d theta and d E_L / d theta for theta = (alpha_1..alpha_norb, b, d)
  * on a batch of configurations (getderpsi/getderbcs psislaterbcs.f90:449-524 with derbeta
- * betafun.f90:481-612; the reference computes no d E_L / d theta).  The inverse is
+ * betafun.f90:196-327; the reference computes no d E_L / d theta).  The inverse is
  * recomputed, not stored.  dlogpsi and delocal are [nw][norb+2]; delocal may be NULL. */
 int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double *delocal);
 

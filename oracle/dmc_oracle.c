@@ -171,7 +171,7 @@ int dmco_kshells(double el, int32_t nsh0, int32_t *numsh, double *ak, int32_t ma
 }
 
 /* ---------------------------------------------------------------- beta(r) */
-/* betafun.f90:412-434 beta0 */
+/* betafun.f90:127-149 beta0 */
 static void beta0(const dmco_ctx *c, double r, double *b1, double *b1p, double *b1pp) {
     double x = c->b * r, d = c->d, cc = c->c;
     if (r < c->small) {
@@ -191,7 +191,7 @@ static void beta0(const dmco_ctx *c, double r, double *b1, double *b1p, double *
     *b1p /= d;
     *b1pp /= d;
 }
-/* betafun.f90:436-456 beta */
+/* betafun.f90:151-171 beta */
 void dmco_beta(const dmco_ctx *c, double r, double out[3]) {
     if (r > c->el * 0.5) { out[0] = out[1] = out[2] = 0.0; return; }
     double bet0, t1, t2, bet1, bet1p, bet1pp, bet2, bet2p, bet2pp;
@@ -202,7 +202,7 @@ void dmco_beta(const dmco_ctx *c, double r, double out[3]) {
     out[1] = (r < 1.0e-14) ? 0.0 : bet1p - bet2p;
     out[2] = bet1pp + bet2pp;
 }
-/* betafun.f90:359-410 initbeta */
+/* betafun.f90:74-125 initbeta */
 static void initbeta(dmco_ctx *c, double bb, double db) {
     double b = bb, d = db, el = c->el;
     c->b = b;
@@ -250,12 +250,12 @@ static void initbeta(dmco_ctx *c, double bb, double db) {
         if (i != 0) c->d2bt[i] += 2.0 * c->dbt[i] / r;
         else c->d2bt[i] = 3.0 * c->d2bt[i];
     }
-    /* setbeta bfact scaling (betafun.f90:347-349) */
+    /* setbeta bfact scaling (betafun.f90:62-64) */
     double f = fabs(c->p.bfact);
     for (int i = 0; i <= nt; i++) { c->bt[i] *= f; c->dbt[i] *= f; c->d2bt[i] *= f; }
 }
 
-/* 4-point Lagrange interpolation weights (coshpot.f90:172-179, betafun.f90:468-475) */
+/* 4-point Lagrange interpolation weights (coshpot.f90:172-179, betafun.f90:183-190) */
 static inline int lagrange(double s, int ntab, double w[4]) {
     int idx = (int)s;
     if (idx > ntab - 2) idx = ntab - 2;
@@ -271,7 +271,7 @@ static inline double interp(const double *t, int idx, const double w[4]) {
     return w[0] * t[idx - 1] + w[1] * t[idx] + w[2] * t[idx + 1] + w[3] * t[idx + 2];
 }
 
-/* betafun.f90:458-479 radialfun */
+/* betafun.f90:173-194 radialfun */
 void dmco_radial(const dmco_ctx *c, double r, double out[3]) {
     if (c->nobeta || r > c->rangeb) { out[0] = out[1] = out[2] = 0.0; return; }
     double w[4];
@@ -281,9 +281,9 @@ void dmco_radial(const dmco_ctx *c, double r, double out[3]) {
     out[2] = interp(c->d2bt, idx, w);
 }
 
-/* betafun.f90:481-612 derbeta: d beta / d(b, d) of the closed form of beta (betafun.f90:
- * 436-456 composed of beta0, 412-434, large-x branch), including the dependence of the
- * constant c on (b, d) (betafun.f90:365-368).  The reference's expression is machine
+/* betafun.f90:196-327 derbeta: d beta / d(b, d) of the closed form of beta (betafun.f90:
+ * 151-171 composed of beta0, 127-149, large-x branch), including the dependence of the
+ * constant c on (b, d) (betafun.f90:80-83).  The reference's expression is machine
  * generated; here the same derivative is taken exactly by forward-mode dual numbers. */
 typedef struct { double v, b, d; } dual;
 static inline dual dc(double v) { return (dual){v, 0.0, 0.0}; }
@@ -335,7 +335,7 @@ static double vpot_v0(const dmco_ctx *c, double v0, double r) {
 }
 double dmco_vpot(const dmco_ctx *c, double r) { return vpot_v0(c, c->p.v0, r); }
 
-/* rungekutta.f90:378-401 rk3d */
+/* rungekutta.f90:127-150 rk3d */
 static void rk3d(const dmco_ctx *c, double v0, double *psi, double *dpsi, double ene,
                  double hb, double dr, int nt, int *ncross) {
     psi[0] = 0.0;
@@ -357,7 +357,7 @@ static void rk3d(const dmco_ctx *c, double v0, double *psi, double *dpsi, double
         if (psi[i] != copysign(psi[i], psi[i - 1])) (*ncross)++;
     }
 }
-/* rungekutta.f90:259-351 orbital (3D branch, nodes = 0) */
+/* rungekutta.f90:8-100 orbital (3D branch, nodes = 0) */
 static int orbital(const dmco_ctx *c, double v0, double *psi, double *dpsi, double *d2psi,
                    double hb, int nt, double dr) {
     int ncross;
@@ -478,7 +478,7 @@ dmco_ctx *dmco_create(const dmco_params *p) {
     c->bt = blk + 8 * (nt + 1); c->dbt = blk + 9 * (nt + 1); c->d2bt = blk + 10 * (nt + 1);
     for (int i = 0; i < p->norb; i++) c->vorb[i] = p->vorb[i];
     if (kstates3d(c, p->norb)) { dmco_destroy(c); return NULL; }
-    /* betafun.f90:297-357 setbeta (bfact >= 0 branch) */
+    /* betafun.f90:12-72 setbeta (bfact >= 0 branch) */
     c->nobeta = p->bfact == 0.0;
     c->rangeb = 0.5 * c->el;
     c->scaleb = nt / c->rangeb;

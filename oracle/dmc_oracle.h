@@ -35,12 +35,12 @@ void dmco_destroy(dmco_ctx *c);
 void dmco_info(const dmco_ctx *c, double out[8]);
 /* k-vectors: ak[nk][3], ak2[nk], numsh[norb] (kshell.f90:20-156) */
 int dmco_kvec(const dmco_ctx *c, double *ak, double *ak2, int32_t *numsh);
-/* tables [0..ntab]: v, u, du, d2u (coshpot.f90:77-133) and b, db, d2b (betafun.f90:359-410) */
+/* tables [0..ntab]: v, u, du, d2u (coshpot.f90:77-133) and b, db, d2b (betafun.f90:74-125) */
 void dmco_tables(const dmco_ctx *c, double *vt, double *ut, double *dut, double *d2ut,
                  double *bt, double *dbt, double *d2bt);
-void dmco_radial(const dmco_ctx *c, double r, double out[3]);   /* betafun.f90:458-479 */
-void dmco_beta(const dmco_ctx *c, double r, double out[3]);     /* betafun.f90:436-456 */
-void dmco_derbeta(const dmco_ctx *c, double r, double out[2]);  /* betafun.f90:481-612 */
+void dmco_radial(const dmco_ctx *c, double r, double out[3]);   /* betafun.f90:173-194 */
+void dmco_beta(const dmco_ctx *c, double r, double out[3]);     /* betafun.f90:151-171 */
+void dmco_derbeta(const dmco_ctx *c, double r, double out[2]);  /* betafun.f90:196-327 */
 double dmco_vpot(const dmco_ctx *c, double r);                  /* coshpot.f90:145-152 */
 
 /* wavefunction pieces on one configuration */

@@ -1,7 +1,7 @@
 """Input decks in the reference's positional stdin format.
 
 The order of the lines follows the reads of the reference: dmcbcs.f90:43-66 (run header),
-wavefunction.f90:36-37 (nodes, a2), psislaterbcs.f90:47-68 (pairing), betafun.f90:304-306
+wavefunction.f90:36-37 (nodes, a2), psislaterbcs.f90:47-68 (pairing), betafun.f90:19-21
 (beta), coshpot.f90:35-41 (interaction/Jastrow), operators.f90:33-41 (four table sizes) and,
 for optimisation runs, optimizer.f90:23-27.
 """
