@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,38 @@ template <class T>
 hipError_t dalloc(T **p, size_t n) {
     return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1));
 }
+
+// Device buffer of dmc_opt_accumulate_noalloc: the parameter derivatives O [nw][P] of the
+// adjoint kernel, in a file-scope array instead of the context's scratch (8 MiB per device,
+// held until the process ends; every call on a device shares it).  Host code must not pass
+// the array's name to a kernel: it uses the device address that hipGetSymbolAddress gives
+// for the current device, resolved once per device by dmc_create.
+constexpr size_t kNoAllocDoubles = size_t(1) << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+std::string g_noalloc_err[kMaxDevices];
+
+// the address of the array on device dev (current), resolved on the first call per device
+double *noalloc_scratch_address(int dev);
+}  // namespace
+
+static __device__ __align__(256) double noalloc_scratch[kNoAllocDoubles];
+
+namespace {
+double *noalloc_scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev] && g_noalloc_err[dev].empty()) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p)
+            g_noalloc_err[dev] = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+        else
+            g_noalloc_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
 }  // namespace
 
 struct dmc_ctx {
@@ -83,6 +116,8 @@ struct dmc_ctx {
     std::vector<int> sofq_numsh;
     bool pending = false;  // a propagation whose branch (factor 1) has not run yet
     int nev_pending = 0;   // timed move-kernel chunks of that propagation
+    int device = 0;        // the device current at dmc_create
+    double *noalloc = nullptr;  // that device's address of noalloc_scratch
 
     template <class T>
     int alloc(T **p, size_t n) {
@@ -373,10 +408,13 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         return fail(DMC_ERR_ARG, "all-particle and VMC moves run in FP64 only");
     }
     c->capacity = p->capacity > 0 ? p->capacity : 4 * (p->nwalk > 0 ? p->nwalk : 1);
-    if (hipStreamCreate(&c->st) != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
         delete c;
         return fail(DMC_ERR_HIP, "hipStreamCreate failed");
     }
+    // the no-alloc entry point's device buffer on this device (a failure here only disables
+    // that entry point, which then reports it)
+    c->noalloc = noalloc_scratch_address(c->device);
     fill_devsys(c);
     const int n1 = s.ntab + 1, nq = s.nk - 1;
     double4 *jt, *bt, *btb, *btd, *mt64;
@@ -964,6 +1002,29 @@ int dmc_opt_accumulate(dmc_ctx *c) {
     int e = c->ensure_scratch(sizeof(double) * size_t(nw) * P + 256);
     if (e) return e;
     double *O = static_cast<double *>(c->scratch);
+    LAUNCHCHK(launch_adjoint(c->sys.n, c->ds, c->A.x, nw, c->sys.p.norb, O, nullptr, c->st));
+    LAUNCHCHK(launch_opt_accumulate(c->ds, c->A, O, nw, P, c->opt_acc, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DMC_OK;
+}
+
+int dmc_opt_accumulate_noalloc(dmc_ctx *c) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
+    const int nw = c->nw, P = c->sys.p.norb + 2;
+    if (nw <= 0) return DMC_OK;
+    if (size_t(nw) * P > kNoAllocDoubles)
+        return fail(DMC_ERR_ARG, "dmc_opt_accumulate_noalloc: " + std::to_string(nw) +
+                                     " walkers x " + std::to_string(P) + " parameters exceed its " +
+                                     std::to_string(kNoAllocDoubles) + "-double device buffer");
+    if (!c->noalloc) {
+        const std::string why = c->device >= 0 && c->device < kMaxDevices
+                                    ? g_noalloc_err[c->device] : std::string("device index");
+        return fail(DMC_ERR_HIP, "dmc_opt_accumulate_noalloc: no device buffer (" + why + ")");
+    }
+    // O is written for every walker by the adjoint kernel before the sums read it: nothing
+    // in the buffer needs clearing
+    double *O = c->noalloc;
     LAUNCHCHK(launch_adjoint(c->sys.n, c->ds, c->A.x, nw, c->sys.p.norb, O, nullptr, c->st));
     LAUNCHCHK(launch_opt_accumulate(c->ds, c->A, O, nw, P, c->opt_acc, c->st));
     HIPCHK(hipStreamSynchronize(c->st));

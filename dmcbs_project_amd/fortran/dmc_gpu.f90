@@ -122,6 +122,11 @@ module dmc_gpu
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      integer(c_int) function dmc_opt_accumulate_noalloc(ctx) &
+            bind(c, name='dmc_opt_accumulate_noalloc')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
       integer(c_int) function dmc_opt_sums(ctx, sums, reset) bind(c, name='dmc_opt_sums')
          import :: c_int, c_ptr, c_int32_t, c_double
          type(c_ptr), value :: ctx

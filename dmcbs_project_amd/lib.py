@@ -61,6 +61,7 @@ SIGNATURES = {
     "dmc_acceptance": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "dmc_adjoint": [_vp, C.c_int32, _f64, _f64, _vp],
     "dmc_opt_accumulate": [_vp],
+    "dmc_opt_accumulate_noalloc": [_vp],
     "dmc_opt_sums": [_vp, _f64, C.c_int32],
     "dmc_gofr": [_vp, C.c_int32, _f64, _f64, _f64, _f64, _f64],
     "dmc_sofq_kvec": [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _vp, _vp],

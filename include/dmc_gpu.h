@@ -142,6 +142,14 @@ int dmc_adjoint(dmc_ctx *c, int32_t nw, const double *x, double *dlogpsi, double
  * sum O_p E (P), sum O_p O_q (P*P, row-major)]; reset = 1 zeroes them afterwards.  The
  * update itself (updatepar, optimizer.f90:114-169: a P x P solve) is host arithmetic. */
 int dmc_opt_accumulate(dmc_ctx *c);
+/* dmc_opt_accumulate without any device allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and summation order, with the
+ * parameter derivatives [nw][norb + 2] in a fixed 8 MiB device buffer of the library
+ * (2^20 doubles, one per device, held until the process ends): populations with
+ * nw * (norb + 2) > 2^20 are refused with DMC_ERR_ARG.  Calls that overlap on one device
+ * (contexts on different host threads or streams) share that buffer and race. */
+int dmc_opt_accumulate_noalloc(dmc_ctx *c);
 int dmc_opt_sums(dmc_ctx *c, double *sums, int32_t reset);
 
 /* Observables of the resident population (the operators of operators.f90:92-108, called
