@@ -291,6 +291,20 @@ class Engine:
         L.check(self.lib.dmc_nofk(self.h, int(ntab), *acc))
         return acc
 
+    def move_trace(self, enable=True):
+        """record every single-particle move of the following steps (dmc_move_trace)"""
+        L.check(self.lib.dmc_move_trace(self.h, int(bool(enable))))
+
+    def read_move_trace(self):
+        """records of the last propagation [nw][npart][12] (dmc_read_move_trace): ratio,
+        Jastrow log-ratio, accepted, draw, proposed position (3), drift (3), log Metropolis
+        ratio"""
+        nw = C.c_int32()
+        L.check(self.lib.dmc_read_move_trace(self.h, 0, C.byref(nw), None))
+        rec = np.zeros((nw.value, self.npart, 12))
+        L.check(self.lib.dmc_read_move_trace(self.h, nw.value, C.byref(nw), rec.ctypes.data))
+        return rec
+
     def kernel_timing(self, enable=True):
         L.check(self.lib.dmc_kernel_timing(self.h, int(bool(enable))))
 

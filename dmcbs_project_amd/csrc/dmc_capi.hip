@@ -117,6 +117,8 @@ struct dmc_ctx {
     bool pending = false;  // a propagation whose branch (factor 1) has not run yet
     int nev_pending = 0;   // timed move-kernel chunks of that propagation
     int device = 0;        // the device current at dmc_create
+    double *trace = nullptr;  // move records (dmc_move_trace), [capacity][npart][kTraceRec]
+    int trace_nw = 0;         // walkers of the last recorded propagation
     double *noalloc = nullptr;  // that device's address of noalloc_scratch
 
     template <class T>
@@ -492,6 +494,7 @@ int dmc_destroy(dmc_ctx *c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->trace) (void)hipFree(c->trace);
     if (c->h_nt) (void)hipHostFree(c->h_nt);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -670,6 +673,7 @@ int dmc_propagate(dmc_ctx *c, int32_t opcomp) {
         // moves in chunks of krefresh, an FP64 refresh of the inverse between chunks, then
         // the end-of-sweep FP64 evaluation; A.sinv serves as the working inverse
         const int np = c->sys.npart;
+        if (c->trace) c->trace_nw = n;
         const int k = c->sys.p.krefresh > 0 ? c->sys.p.krefresh : np;
         for (int m0 = 0; m0 < np; m0 += k) {
             const int m1 = m0 + k < np ? m0 + k : np;
@@ -968,6 +972,36 @@ int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *w
         enknow[k] += sums[1 + k];
         enkerr[k] += sums[1 + nsh + k];
     }
+    return DMC_OK;
+}
+
+int dmc_move_trace(dmc_ctx *c, int32_t enable) {
+    if (!c) return fail(DMC_ERR_ARG, "null context");
+    if (c->sys.p.movetype != DMC_SINGLEPARTICLE)
+        return fail(DMC_ERR_ARG, "move records need single-particle moves");
+    if (c->pending) return fail(DMC_ERR_ARG, "a propagation awaits dmc_branch");
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (!enable) {
+        if (c->trace) HIPCHK(hipFree(c->trace));
+        c->trace = nullptr;
+    } else if (!c->trace) {
+        const size_t n = size_t(c->capacity) * c->sys.npart * kTraceRec;
+        HIPCHK(hipMalloc(reinterpret_cast<void **>(&c->trace), sizeof(double) * n));
+    }
+    c->T.trace = c->trace;
+    c->trace_nw = 0;
+    return DMC_OK;
+}
+
+int dmc_read_move_trace(dmc_ctx *c, int32_t cap, int32_t *nw, double *rec) {
+    if (!c || !nw) return fail(DMC_ERR_ARG, "null argument");
+    if (!c->trace) return fail(DMC_ERR_ARG, "move records are off (dmc_move_trace)");
+    *nw = c->trace_nw;
+    if (!rec) return DMC_OK;
+    if (cap < c->trace_nw) return fail(DMC_ERR_ARG, "output capacity smaller than the population");
+    const size_t n = size_t(c->trace_nw) * c->sys.npart * kTraceRec;
+    HIPCHK(hipMemcpyAsync(rec, c->trace, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     return DMC_OK;
 }
 

@@ -59,6 +59,13 @@ struct DevStep {
     int64_t *hist;        // branch multiplicity histogram [500]
     int *nacc;            // accepted single-particle moves per walker
     int64_t *moves;       // accepted and proposed single-particle moves (running totals)
+    // record of every single-particle move, [w][npart][kTraceRec], or null (not recorded)
+    double *trace;
 };
+
+// fields of a move record: determinant ratio, Jastrow log-ratio, accepted (0/1), the
+// acceptance draw, the proposed position (3), the drift after the cut (3), the log of the
+// Metropolis ratio (NaN for a move the node test refuses), 0
+constexpr int kTraceRec = 12;
 
 }  // namespace dmc

@@ -584,6 +584,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
         }
         const R dn[3] = {dn0, dn1, dn2};
         bool accept = !(S.fn && !(rho > 0.0)) && rho != 0.0;
+        double logp_rec = __builtin_nan("");
         if (accept) {
             // Metropolis exponent of the drift-diffusion Green's functions: FP64 moves in FP64,
             // mixed moves in FP32 (its inputs carry FP32 errors already)
@@ -595,6 +596,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                     lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
                 }
                 const double logp = 2.0 * (log(fabs(rho)) + dU) + lq * half_isig2;
+                logp_rec = logp;
                 accept = u4 < exp(fmin(0.0, logp));
             } else {
                 const float fs2 = float(sig2), fcut = float(dcut);
@@ -606,8 +608,23 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                 }
                 const float logp = 2.0f * (log_r<float>(fabsf(float(rho))) + dU) +
                                    lq * float(half_isig2);
+                logp_rec = logp;
                 accept = u4 < double(exp_r<float>(fminf(0.0f, logp)));
             }
+        }
+        if (T.trace && tid == 0) {
+            double *rec = T.trace + (size_t(w) * S.npart + ip) * kTraceRec;
+            rec[0] = rho;
+            rec[1] = double(dU);
+            rec[2] = accept ? 1.0 : 0.0;
+            rec[3] = u4;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                rec[4 + d] = xn[d];
+                rec[7 + d] = double(dro[d]);
+            }
+            rec[10] = logp_rec;
+            rec[11] = 0.0;
         }
         if (accept) {
             ++nacc;

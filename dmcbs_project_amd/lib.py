@@ -68,6 +68,8 @@ SIGNATURES = {
     "dmc_sofq": [_vp, C.c_int32, _f64, _f64, _f64],
     "dmc_nofk_shells": [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _vp, _vp],
     "dmc_nofk": [_vp, C.c_int32, _f64, _f64, _f64],
+    "dmc_move_trace": [_vp, C.c_int32],
+    "dmc_read_move_trace": [_vp, C.c_int32, C.POINTER(C.c_int32), _vp],
     "dmc_kernel_timing": [_vp, C.c_int32],
     "dmc_kernel_times": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32],
     "dmc_sync": [_vp],

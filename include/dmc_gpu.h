@@ -201,6 +201,17 @@ int dmc_nofk(dmc_ctx *c, int32_t ntab, double *enknow, double *enkerr, double *w
  * Returns the summed milliseconds and the number of launches since the last reset. */
 int dmc_kernel_timing(dmc_ctx *c, int32_t enable);
 int dmc_kernel_times(dmc_ctx *c, double *ms, int64_t *launches, int32_t reset);
+/* Record of every single-particle move of the following propagations: no counterpart in the
+ * reference, whose step1 keeps none; it is how the mixed-precision ratios are compared with
+ * FP64 on the same configuration and proposal.  enable = 1 allocates [capacity][npart][12]
+ * doubles on the device, 0 frees them.  Recording costs one store per move. */
+int dmc_move_trace(dmc_ctx *c, int32_t enable);
+/* The records of the last propagation, walkers in their order before branching (the order
+ * of dmc_read_walkers before dmc_propagate): rec[w][ip][12] = determinant ratio, Jastrow
+ * log-ratio, accepted (0/1), acceptance draw, proposed position (3), drift after the cut
+ * (3), log of the Metropolis ratio (NaN when the node test refused the move), 0.
+ * *nw = that population; rec may be NULL (then only *nw), else cap >= *nw. */
+int dmc_read_move_trace(dmc_ctx *c, int32_t cap, int32_t *nw, double *rec);
 /* Synchronise the context's stream. */
 int dmc_sync(dmc_ctx *c);
 

@@ -328,7 +328,7 @@ contains
    integer(kind=i4) :: istin,istout,n1,nt,i,j,k,jj,ist
    integer(kind=i8) :: ndump,nin,kk
    integer(kind=i8), allocatable :: dumps(:)
-   real(kind=r8) :: rn(1),dummy,tau,valn,errn,val,err,ene
+   real(kind=r8) :: rn(1),dummy,tau,valn,errn,val,err,ene,timebl0,timebl
    real(kind=r8), allocatable :: xtemp(:)
    real(kind=r8), pointer :: par(:)
    integer(kind=i4) :: npar
@@ -390,6 +390,9 @@ contains
    write (21) int(nav*nstep,i8)
    if (any(dumps.eq.0)) call dump_stack(istin,istout)
    do i=1,nav
+! the block timer of dmcbcs.f90:434,450,487 (process CPU time around the block's steps),
+! printed in the reference's format
+      call cpu_time(timebl0)
       do j=1,nstep
          jj=jj+1
          call step1(istin,istout,j.eq.nstep)
@@ -405,6 +408,9 @@ contains
          tau=tau+dt
          if (any(dumps.eq.jj)) call dump_stack(istin,istout)
       enddo
+      call cpu_time(timebl)
+      write (6,'(''Time for block (sec) (min) = '',t35,2f11.3)') timebl-timebl0, &
+         (timebl-timebl0)/60.0_r8
       call update
       do k=1,14
          call result(k,valn,errn,val,err)
