@@ -122,6 +122,19 @@ module dmc_gpu
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      integer(c_int) function dmc_move_trace(ctx, enable) bind(c, name='dmc_move_trace')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: enable
+      end function
+      integer(c_int) function dmc_read_move_trace(ctx, cap, nw, rec) &
+            bind(c, name='dmc_read_move_trace')
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: cap
+         integer(c_int32_t), intent(out) :: nw
+         real(c_double), intent(out) :: rec(*)
+      end function
       integer(c_int) function dmc_opt_accumulate_noalloc(ctx) &
             bind(c, name='dmc_opt_accumulate_noalloc')
          import :: c_int, c_ptr

@@ -17,6 +17,8 @@ program dmcbcs_gpu
    use estimator
    use optimizer
    use operators
+   use random
+   use random2
    implicit none
    integer(kind=i8) :: irn,irn0
    integer(kind=i4) :: npart,nwalk,idmc,neq,nav,nstep,ndim,ntab,nbin(2)
@@ -28,7 +30,8 @@ program dmcbcs_gpu
    logical :: isite,units,fn,euclid,empty,operat
    character(len=90), allocatable :: answer(:)
    character(len=32) :: arg
-   real(kind=r8) :: timebl0,timebl
+   real(kind=r8) :: timebl0,timebl,rn(1),dummy
+   real(kind=r8), allocatable :: xtemp(:)
 
    prec=DMC_FP64
    movet=DMC_ALLPARTICLE
@@ -135,12 +138,33 @@ program dmcbcs_gpu
    call addest(14,'sign(wtg)-weighted =')
    endif
    call create(2,int(10*nwalk,i4),npart,ndim,euclid)
+   allocate(xtemp(ndim))
 
 ! initial walkers (dmcbcs.f90:237-301): the reference's generation from one random
 ! configuration, on the host side of the engine, or the walker file
    if (isite) then
       write (6,'(''Initial walkers from random configs'')')
-      call dmc_check(dmc_init_walkers(engine_ctx(),irn0,nwalk),'initial walkers')
+! one configuration from the module stream, then per walker npart random swaps and the next
+! ran2 state of the walker stream, as dmcbcs.f90:240-286 generates them
+      irn=irn0
+      call setrn(irn)
+      w1%x=reshape(el*(randn(size(w1%x))-0.5_r8),shape(w1%x))
+      do k=1,nwalk
+         do i=1,npart
+            rn=randn(1)
+            j=npart*rn(1)+1
+            xtemp(:)=w1%x(:,i)
+            w1%x(:,i)=w1%x(:,j)
+            w1%x(:,j)=xtemp(:)
+         enddo
+         w1%weight=1.0_r8
+         w1%irn=irn
+         call ran2(dummy,w1%irn)
+         irn=w1%irn
+         w1%x0=0.0_r8
+         call push(1,w1)
+      enddo
+      call flush_stack
       tau0=0.0_r8
    else
       write (6,'(''Initial walkers from file'')')
@@ -149,7 +173,7 @@ program dmcbcs_gpu
       write (6,'(''Number of walkers ='',t40,i10)') n1
       do k=1,n1
          read (9,'(6e15.7)') w1%x
-         read (9,'(e15.7)') w1%weight
+         read (9,'(e15.7,2e25.15)') w1%weight,w1%psil0,w1%psigl0
          read (9,'(i20)') w1%irn
          call push(1,w1)
       enddo

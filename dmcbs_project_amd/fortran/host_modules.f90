@@ -4,10 +4,13 @@
 !   potential    setphi                          (coshpot.f90:14-143)
 !   wavefunction setpsi, hpsi, getderpsi, setdetparam, getdetparam, move1
 !                                                (wavefunction.f90:13-404)
+!   random2      ran1, ran2                                  (ran.f90:1-50)
+!   random       randn, gaussian, setrn, savern              (ran.f90:52-124)
 !   stack        type(walker), create, push, pop, numstack   (stack.f90:6-119)
 !   step         setstep, step1, branch, writewtisto         (step.f90:11-334)
 !   estimator    setestnum, addest, zerest, update, resstring (estimator.f90:19-149)
-!   optimizer    setupopt, calder, updatepar                  (optimizer.f90:14-169)
+!   optimizer    setupopt, calder(w, ene0) and calder over the population, updatepar
+!                                                (optimizer.f90:14-169)
 !   gofr, sofq, pdistribution   setup, zero, add, update, write of g(r), S(q), n(k)
 !                (gofr.f90:14-177, sofq.f90:14-119, momentum3d.f90:13-190)
 !   operators    setoperators, zero/add/update/writeoperators (operators.f90:7-140, g(r),
@@ -81,23 +84,120 @@ contains
    end subroutine setphi
 end module potential
 
+! The reference's random-number modules for host code (walker generation, dmcbcs.f90:237-301):
+! the 48-bit linear congruential generators ran1 and ran2 (ran.f90:8-50) and the module
+! stream with randn, gaussian (Box-Muller), setrn and savern (ran.f90:52-124).  The device
+! kernels draw from the same generators with the same constants.
+module random2
+   implicit none
+   integer, private, parameter :: i8=selected_int_kind(15)
+   integer, private, parameter :: r8=selected_real_kind(15,9)
+   integer(kind=i8), private, parameter :: m24=2_i8**24-1_i8, m48=2_i8**48-1_i8
+contains
+! s <- (a s + c) mod 2^48 with the product split in 24-bit halves so that no intermediate
+! exceeds 63 bits: a s = a_lo s_lo + 2^24 (a_lo s_hi + a_hi s_lo) mod 2^48
+   subroutine lcg48(s,a,c)
+   integer(kind=i8) :: s,a,c
+   integer(kind=i8) :: slo,shi,alo,ahi,mid
+   slo=iand(s,m24)
+   shi=iand(ishft(s,-24),m24)
+   alo=iand(a,m24)
+   ahi=iand(ishft(a,-24),m24)
+   mid=iand(alo*shi+ahi*slo,m24)
+   s=iand(alo*slo+ishft(mid,24)+c,m48)
+   end subroutine lcg48
+
+   subroutine ran1(rn,irn)
+   real(kind=r8) :: rn
+   integer(kind=i8) :: irn
+   call lcg48(irn,44485709377909_i8,96309754297_i8)
+   rn=real(ior(irn,1_i8),r8)*2.0_r8**(-48)
+   end subroutine ran1
+
+   subroutine ran2(rn,irn)
+   real(kind=r8) :: rn
+   integer(kind=i8) :: irn
+   call lcg48(irn,34522712143931_i8,55789347517_i8)
+   rn=real(ior(irn,1_i8),r8)*2.0_r8**(-48)
+   end subroutine ran2
+end module random2
+
+module random
+   use random2, only : ran1
+   implicit none
+   integer, private, parameter :: i4=selected_int_kind(9)
+   integer, private, parameter :: i8=selected_int_kind(15)
+   integer, private, parameter :: r8=selected_real_kind(15,9)
+   integer(kind=i8), private, save :: stream=1_i8
+contains
+! n uniform variates from the module stream
+   function randn(n)
+   integer(kind=i4) :: n
+   real(kind=r8) :: randn(n)
+   integer(kind=i4) :: i
+   do i=1,n
+      call ran1(randn(i),stream)
+   enddo
+   end function randn
+
+! n gaussian variates: 2 ceil(n/2) uniforms drawn first, then per pair (u1, u2)
+! sin(2 pi u1) r and cos(2 pi u1) r with r = sqrt(-2 ln u2); an odd last variate takes the
+! cosine of its pair
+   function gaussian(n)
+   integer(kind=i4) :: n
+   real(kind=r8) :: gaussian(n)
+   real(kind=r8) :: u(2*((n+1)/2)),arg,r
+   integer(kind=i4) :: i
+   u=randn(size(u))
+   do i=1,(n+1)/2
+      arg=2.0_r8*(4.0_r8*atan(1.0_r8))*u(2*i-1)
+      r=sqrt(-2.0_r8*log(u(2*i)))
+      if (2*i.le.n) then
+         gaussian(2*i-1)=sin(arg)*r
+         gaussian(2*i)=cos(arg)*r
+      else
+         gaussian(2*i-1)=cos(arg)*r
+      endif
+   enddo
+   end function gaussian
+
+   subroutine setrn(irnin)
+   integer(kind=i8) :: irnin
+   stream=iand(irnin,2_i8**48-1_i8)
+   end subroutine setrn
+
+   subroutine savern(irnout)
+   integer(kind=i8) :: irnout
+   irnout=stream
+   end subroutine savern
+end module random
+
 module stack
    use engine
    implicit none
+! the walker record of stack.f90:6-13, field for field (psigl, d2psig, dpsig and isg describe
+! the guiding function, equal to psi here since a2 = 0; vext is 0 without an external
+! potential; psil0 and psigl0 are the values read from a walker file; wt0 is declared
+! real(kind=i4), i.e. default real, as the reference declares it)
    type walker
-      real(kind=r8), allocatable :: x(:,:),dpsi(:,:)
-      real(kind=r8) :: psil=0.0_r8,d2psi=0.0_r8,v=0.0_r8,weight=1.0_r8,wt0=1.0_r8
-      integer(kind=i4) :: is=1
+      real(kind=r8) :: psil=0.0_r8,d2psi=0.0_r8,v=0.0_r8,weight=1.0_r8,psigl=0.0_r8, &
+         d2psig=0.0_r8,psil0=0.0_r8,psigl0=0.0_r8,vext=0.0_r8
+      real(kind=r8), pointer :: x(:,:)=>null(),dpsi(:,:)=>null(),dpsig(:,:)=>null()
+      integer(kind=i4) :: is=1,isg=1
       integer(kind=i8) :: irn=0
+      real(kind=r8), pointer :: x0(:,:)=>null()
+      real(kind=i4) :: wt0=1.0
    end type walker
    type(walker), save :: w1,w2
    integer(kind=i4), private, save :: npart,ndim,cap
-! host side of the stacks: walkers pushed before the next device call, or downloaded for
-! popping.  Stack numbers select nothing: the device holds the one population.
-   real(kind=r8), allocatable, target, private, save :: hx(:,:,:),hd(:,:,:),hsc(:,:)
-   integer(kind=i8), allocatable, target, private, save :: hirn(:)
-   integer(kind=i4), private, save :: nhost=0
-   logical, private, save :: downloaded=.false.
+! Host side of the reference's two stacks (stack.f90:15-17): walkers pushed onto stack i
+! wait in host arrays until the next device call loads them (flush_stack); popping stack i
+! when it is empty on the host downloads the device population into it first.  So a loop
+! of dmcbcs.f90's form (pop stack istin, work on the walker, push stack istout) runs on the
+! host, and the device holds the population between such loops.
+   real(kind=r8), allocatable, target, private, save :: hx(:,:,:,:),hd(:,:,:,:),hsc(:,:,:)
+   integer(kind=i8), allocatable, target, private, save :: hirn(:,:)
+   integer(kind=i4), private, save :: nhost(2)=0
 contains
    subroutine create(nst,nw,np,nd,euclid)
    integer(kind=i4) :: nst,nw,np,nd
@@ -105,82 +205,125 @@ contains
    npart=np
    ndim=nd
    cap=nw
-   allocate(hx(nd,np,nw),hd(nd,np,nw),hsc(6,nw),hirn(nw))
-   allocate(w1%x(nd,np),w1%dpsi(nd,np),w2%x(nd,np),w2%dpsi(nd,np))
+   allocate(hx(nd,np,nw,2),hd(nd,np,nw,2),hsc(6,nw,2),hirn(nw,2))
+   call walker_arrays(w1)
+   call walker_arrays(w2)
    nhost=0
    end subroutine create
 
-! push (stack.f90:83-95): the walker joins the population at the next device call
+   subroutine walker_arrays(w)
+   type(walker) :: w
+   if (associated(w%x)) return
+   allocate(w%x(ndim,npart),w%dpsi(ndim,npart),w%dpsig(ndim,npart),w%x0(ndim,npart))
+   w%x=0.0_r8
+   w%dpsi=0.0_r8
+   w%dpsig=0.0_r8
+   w%x0=0.0_r8
+   end subroutine walker_arrays
+
+! push (stack.f90:83-95): onto host stack i; it joins the device population at the next
+! device call
    subroutine push(i,w)
    integer(kind=i4) :: i
    type(walker) :: w
-   if (downloaded) then
-      nhost=0
-      downloaded=.false.
-   endif
-   if (nhost.ge.cap) error stop 'stack overflow'
-   nhost=nhost+1
-   hx(:,:,nhost)=w%x
-   hirn(nhost)=w%irn
+   integer(kind=i4) :: k
+   if (i.lt.1.or.i.gt.2) error stop 'push: stack number'
+   if (nhost(i).ge.cap) error stop 'stack overflow'
+   k=nhost(i)+1
+   nhost(i)=k
+   hx(:,:,k,i)=w%x
+   hd(:,:,k,i)=w%dpsi
+   hirn(k,i)=w%irn
+   hsc(:,k,i)=(/w%weight,w%psil,w%d2psi,w%v,real(w%is,r8),real(w%wt0,r8)/)
    end subroutine push
 
-! pop (stack.f90:97-112): the device population is downloaded on the first pop
+! pop (stack.f90:97-112): from host stack i, which takes the device population when it is
+! empty on the host
    subroutine pop(i,w,empty)
    integer(kind=i4) :: i
    type(walker) :: w
    logical :: empty
    integer(c_int32_t) :: nw
-   if (.not.downloaded.and.nhost.eq.0.and.started) then
-      call dmc_check(dmc_read_walkers(ctx,cap,nw,c_loc(hx),c_loc(hd),c_loc(hsc),c_loc(hirn)), &
-         'pop')
-      nhost=nw
-      downloaded=.true.
-      call dmc_check(dmc_load_walkers(ctx,0_c_int32_t,hx,hirn),'pop')
+   integer(kind=i4) :: k
+   if (i.lt.1.or.i.gt.2) error stop 'pop: stack number'
+   if (nhost(i).eq.0.and.started) then
+      call dmc_check(dmc_read_walkers(ctx,cap,nw,c_loc(hx(1,1,1,i)),c_loc(hd(1,1,1,i)), &
+         c_loc(hsc(1,1,i)),c_loc(hirn(1,i))),'pop')
+      nhost(i)=nw
+      if (nw.gt.0) call dmc_check(dmc_load_walkers(ctx,0_c_int32_t,hx(:,:,:,i),hirn(:,i)),'pop')
    endif
-   empty=nhost.eq.0
+   empty=nhost(i).eq.0
    if (empty) return
-   if (.not.allocated(w%x)) allocate(w%x(ndim,npart),w%dpsi(ndim,npart))
-   w%x=hx(:,:,nhost)
-   w%irn=hirn(nhost)
-   if (downloaded) then
-      w%dpsi=hd(:,:,nhost)
-      w%weight=hsc(1,nhost)
-      w%psil=hsc(2,nhost)
-      w%d2psi=hsc(3,nhost)
-      w%v=hsc(4,nhost)
-      w%is=nint(hsc(5,nhost))
-      w%wt0=hsc(6,nhost)
-   endif
-   nhost=nhost-1
+   k=nhost(i)
+   call walker_arrays(w)
+   w%x=hx(:,:,k,i)
+   w%irn=hirn(k,i)
+   w%dpsi=hd(:,:,k,i)
+   w%weight=hsc(1,k,i)
+   w%psil=hsc(2,k,i)
+   w%d2psi=hsc(3,k,i)
+   w%v=hsc(4,k,i)
+   w%is=nint(hsc(5,k,i))
+   w%wt0=real(hsc(6,k,i))
+! guiding function = psi (a2 = 0); the device keeps no x0 (no euclidean estimators)
+   w%dpsig=w%dpsi
+   w%psigl=w%psil
+   w%d2psig=w%d2psi
+   w%isg=w%is
+   w%psil0=w%psil
+   w%psigl0=w%psil
+   w%vext=0.0_r8
+   w%x0=0.0_r8
+   nhost(i)=k-1
    end subroutine pop
 
-! walkers pushed on the host are loaded onto the device (initial hpsi there)
+! walkers pushed on the host are loaded onto the device, stack 1 then stack 2 (positions and
+! random states; the device evaluates hpsi and starts them with weight 1, wt0 = sign)
    subroutine flush_stack
    type(c_ptr) :: c
+   integer(kind=i4) :: i,n
+   integer(c_int32_t) :: nd
+   real(kind=r8), allocatable, target :: xa(:,:,:)
+   integer(kind=i8), allocatable, target :: ra(:)
    c=engine_ctx()
-   if (nhost.gt.0.and..not.downloaded) then
-      call dmc_check(dmc_load_walkers(c,nhost,hx,hirn),'load walkers')
-      nhost=0
-   endif
+   n=sum(nhost)
+   if (n.eq.0) return
+   nd=0
+   call dmc_check(dmc_population(c,nd),'load walkers')
+   allocate(xa(ndim,npart,n+nd),ra(n+nd))
+! walkers already on the device stay (downloaded and loaded again with the new ones)
+   if (nd.gt.0) call dmc_check(dmc_read_walkers(c,int(n+nd,c_int32_t),nd,c_loc(xa), &
+      c_null_ptr,c_null_ptr,c_loc(ra)),'load walkers')
+   do i=1,2
+      if (nhost(i).gt.0) then
+         xa(:,:,nd+1:nd+nhost(i))=hx(:,:,1:nhost(i),i)
+         ra(nd+1:nd+nhost(i))=hirn(1:nhost(i),i)
+         nd=nd+nhost(i)
+         nhost(i)=0
+      endif
+   enddo
+   call dmc_check(dmc_load_walkers(c,nd,xa,ra),'load walkers')
    end subroutine flush_stack
 
 ! the resident walkers evaluated again with the current parameters (positions and random
 ! states kept; weight 1, wt0 = sign), as the set-up pass of dmcbcs.f90:314-353
    subroutine reload_stack
    integer(c_int32_t) :: nw
+   call flush_stack
    call dmc_check(dmc_read_walkers(engine_ctx(),cap,nw,c_loc(hx),c_null_ptr,c_null_ptr, &
       c_loc(hirn)),'reload')
-   call dmc_check(dmc_load_walkers(ctx,nw,hx,hirn),'reload')
-   nhost=0
-   downloaded=.false.
+   call dmc_check(dmc_load_walkers(ctx,nw,hx(:,:,:,1),hirn(:,1)),'reload')
    end subroutine reload_stack
 
+! walkers on stack i: those waiting on the host plus, for a stack empty on the host, the
+! device population (which the next pop of that stack takes)
    function numstack(i)
    integer(kind=i4) :: numstack,i
    integer(c_int32_t) :: nw
    nw=0
    if (started) call dmc_check(dmc_population(ctx,nw),'numstack')
-   numstack=nw+nhost
+   numstack=nhost(i)
+   if (nhost(i).eq.0) numstack=nw
    end function numstack
 end module stack
 
@@ -244,6 +387,13 @@ contains
    w%d2psi=d2psi(1)
    w%v=v(1)
    w%is=is(1)
+! the guiding-function fields of wavefunction.f90:190-199 with a2 = 0, and no external
+! potential
+   w%psigl=w%psil
+   w%d2psig=w%d2psi
+   if (associated(w%dpsig)) w%dpsig=w%dpsi
+   w%isg=w%is
+   w%vext=0.0_r8
    end subroutine hpsi
 
 ! getderpsi (wavefunction.f90:203-229): d log psi / d(alpha_1..alpha_norb[, b, d]) at x
@@ -410,6 +560,14 @@ module optimizer
    real(kind=r8), private, save :: delta0,eps
    integer(kind=i4), private, save :: nparm
    logical, private, save :: iopt(2)
+! sums of the per-walker calder(w, ene0), in the layout of dmc_opt_sums: n, sum E, sum E^2,
+! sum O_p, sum O_p E, sum O_p O_q (p, q over alpha_1..alpha_norb, b, d)
+   real(kind=r8), allocatable, private, save :: hsums(:)
+! calder(w, ene0) for one walker (optimizer.f90:90-112) and calder with no arguments for
+! every resident walker at once on the device
+   interface calder
+      module procedure calder_walker, calder_population
+   end interface calder
 contains
 ! setupopt (optimizer.f90:14-69): optiter, optimise Jastrow, optimise nodes, delta, eps.
 ! The parameters are the determinant's alpha_1..alpha_norb (getdetparam); the cosh Jastrow
@@ -432,10 +590,37 @@ contains
    end subroutine setupopt
 
 ! calder (optimizer.f90:90-112) for every resident walker at once: the device computes
-! d log psi with the adjoint kernel and accumulates the sums
-   subroutine calder
+! d log psi with the adjoint kernel and accumulates the sums (E = each walker's stored
+! E_L / (npart eunit))
+   subroutine calder_population
    call dmc_check(dmc_opt_accumulate(engine_ctx()),'calder')
-   end subroutine calder
+   end subroutine calder_population
+
+! calder(w, ene0) (optimizer.f90:90-112) for one walker: d log psi at w%x from the adjoint
+! kernel, added with the caller's energy ene0 to the host sums
+   subroutine calder_walker(w,ene0)
+   use stack, only : walker
+   type(walker) :: w
+   real(kind=r8) :: ene0
+   real(kind=r8) :: d(prm%norb+2)
+   integer(kind=i4) :: np,i,j
+   np=prm%norb+2
+   if (.not.allocated(hsums)) then
+      allocate(hsums(3+2*np+np*np))
+      hsums=0.0_r8
+   endif
+   call dmc_check(dmc_adjoint(engine_ctx(),1_c_int32_t,w%x,d,c_null_ptr),'calder')
+   hsums(1)=hsums(1)+1.0_r8
+   hsums(2)=hsums(2)+ene0
+   hsums(3)=hsums(3)+ene0**2
+   do i=1,np
+      hsums(3+i)=hsums(3+i)+d(i)
+      hsums(3+np+i)=hsums(3+np+i)+d(i)*ene0
+      do j=1,np
+         hsums(3+2*np+(i-1)*np+j)=hsums(3+2*np+(i-1)*np+j)+d(i)*d(j)
+      enddo
+   enddo
+   end subroutine calder_walker
 
 ! updatepar (optimizer.f90:114-169): f = -2<O E> + 2<O><E>, S = <O O> - <O><O> + eps,
 ! parameters += delta S^-1 f; the new parameters go to the wavefunction (setdetparam)
@@ -447,6 +632,11 @@ contains
    integer(kind=i4) :: i,j,np
    np=prm%norb+2
    call dmc_check(dmc_opt_sums(engine_ctx(),sums,1_c_int32_t),'updatepar')
+! the sums of both forms of calder
+   if (allocated(hsums)) then
+      sums=sums+hsums
+      hsums=0.0_r8
+   endif
    n=sums(1)
    esum=sums(2)/n
    esum2=sums(3)/n
