@@ -60,25 +60,110 @@ __device__ __forceinline__ void sincos_turns(float t, float &s, float &c) {
     c = __uint_as_float(__float_as_uint(c0) ^ (((iq ^ (iq >> 1)) & 1u) << 31));
 }
 
-// natural log and exp: the library functions in FP64, the hardware log2/exp2 (about 1 ulp)
-// scaled by ln 2 / log2 e in FP32
+// The same in FP64: t - q/4 is exact (Sterbenz), x = 2 pi (t - q/4) in [-pi/4, pi/4] carries
+// one rounding; the Taylor polynomials of sin to x^17 and of cos to x^18 (coefficients
+// +-1/n!, truncation errors below 1e-19) in Horner form in x^2; the quarter turn as above
+__device__ __forceinline__ void sincos_turns(double t, double &s, double &c) {
+    const double q = rint(4.0 * t);
+    const double x = fma(-0.25, q, t) * 6.283185307179586477;
+    const double x2 = x * x;
+    double sp = fma(x2, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);  // 1/17!, -1/15!
+    sp = fma(x2, sp, 1.0 / 6227020800.0);
+    sp = fma(x2, sp, -1.0 / 39916800.0);
+    sp = fma(x2, sp, 1.0 / 362880.0);
+    sp = fma(x2, sp, -1.0 / 5040.0);
+    sp = fma(x2, sp, 1.0 / 120.0);
+    sp = fma(x2, sp, -1.0 / 6.0);
+    const double sn = fma(x * x2, sp, x);
+    double cp = fma(x2, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);  // -1/18!, 1/16!
+    cp = fma(x2, cp, -1.0 / 87178291200.0);
+    cp = fma(x2, cp, 1.0 / 479001600.0);
+    cp = fma(x2, cp, -1.0 / 3628800.0);
+    cp = fma(x2, cp, 1.0 / 40320.0);
+    cp = fma(x2, cp, -1.0 / 720.0);
+    cp = fma(x2, cp, 1.0 / 24.0);
+    cp = fma(x2, cp, -0.5);
+    const double cs = fma(x2, cp, 1.0);
+    const unsigned iq = unsigned(int(q));
+    const bool sw = iq & 1u;
+    const double s0 = sw ? cs : sn, c0 = sw ? sn : cs;
+    s = __longlong_as_double(__double_as_longlong(s0) ^ ((long long)((iq >> 1) & 1u) << 63));
+    c = __longlong_as_double(__double_as_longlong(c0) ^
+                             ((long long)((iq ^ (iq >> 1)) & 1u) << 63));
+}
+
+// natural log of a positive normal FP64 number: x = m 2^e with m in [1/sqrt 2, sqrt 2)
+// (frexp), ln m = 2 atanh(z) = 2 (z + z^3/3 + ... + z^19/19) with z = (m - 1)/(m + 1),
+// |z| <= 0.1716 (truncation below 1e-18), and e ln 2 with ln 2 split in a high part exact in
+// 32 bits and a low part
+__device__ __forceinline__ double log_pos(double x) {
+    int e;
+    double m = frexp(x, &e);  // [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        --e;
+    }
+    const double d = m + 1.0;
+    double r = __builtin_amdgcn_rcp(d);  // about 1 ulp, then two Newton steps
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    const double z = (m - 1.0) * r, z2 = z * z;
+    double p = fma(z2, 1.0 / 19.0, 1.0 / 17.0);
+    p = fma(z2, p, 1.0 / 15.0);
+    p = fma(z2, p, 1.0 / 13.0);
+    p = fma(z2, p, 1.0 / 11.0);
+    p = fma(z2, p, 1.0 / 9.0);
+    p = fma(z2, p, 1.0 / 7.0);
+    p = fma(z2, p, 1.0 / 5.0);
+    p = fma(z2, p, 1.0 / 3.0);
+    const double lm = 2.0 * fma(z * z2, p, z);
+    const double fe = double(e);
+    return fma(fe, 0.693147180369123816490, fma(fe, 1.90821492927058770002e-10, lm));
+}
+
+// exp(y) for y <= 0 in FP64: y = k ln 2 + r with k = rint(y / ln 2), r = y - k ln 2 from the
+// split ln 2 (Cody-Waite), |r| <= ln 2 / 2; exp(r) by its Taylor polynomial to r^13
+// (truncation below 5e-18) and the factor 2^k by ldexp; below -1100 the result is 0
+__device__ __forceinline__ double exp_nonpos(double y) {
+    y = fmax(y, -1100.0);
+    const double k = rint(y * 1.44269504088896340736);
+    const double r = fma(-k, 1.90821492927058770002e-10, fma(-k, 0.693147180369123816490, y));
+    double p = fma(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);  // 1/13!, 1/12!
+    p = fma(r, p, 1.0 / 39916800.0);
+    p = fma(r, p, 1.0 / 3628800.0);
+    p = fma(r, p, 1.0 / 362880.0);
+    p = fma(r, p, 1.0 / 40320.0);
+    p = fma(r, p, 1.0 / 5040.0);
+    p = fma(r, p, 1.0 / 720.0);
+    p = fma(r, p, 1.0 / 120.0);
+    p = fma(r, p, 1.0 / 24.0);
+    p = fma(r, p, 1.0 / 6.0);
+    p = fma(r, p, 0.5);
+    p = fma(r, p, 1.0);
+    p = fma(r, p, 1.0);
+    return ldexp(p, int(k));
+}
+
+// natural log (positive argument) and exp (non-positive argument, as the moves use them):
+// the polynomials above in FP64, the hardware log2/exp2 (about 1 ulp) scaled by ln 2 /
+// log2 e in FP32
 template <typename R>
 __device__ __forceinline__ R log_r(R x) {
-    if constexpr (sizeof(R) == 8) return log(x);
+    if constexpr (sizeof(R) == 8) return log_pos(x);
     else return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
 }
 template <typename R>
 __device__ __forceinline__ R exp_r(R x) {
-    if constexpr (sizeof(R) == 8) return exp(x);
+    if constexpr (sizeof(R) == 8) return exp_nonpos(x);
     else return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
 }
 
 // sin and cos of 2 pi t for a turn count t given in FP64: FP64 reduction to the nearest
-// turn, then FP64 sincos (R = double) or the FP32 polynomials above
+// turn, then the FP64 or FP32 polynomials above
 template <typename R>
 __device__ __forceinline__ void sincos_2pi(double t, R &s, R &c) {
     t -= rint(t);
-    if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * t, &s, &c);
+    if constexpr (sizeof(R) == 8) sincos_turns(t, s, c);
     else sincos_turns(float(t), s, c);
 }
 
@@ -484,7 +569,7 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
             const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
             const double ua = double(sa | 1ull) * 0x1p-48, ub = double(sb | 1ull) * 0x1p-48;
             R sn, cs;
-            if constexpr (sizeof(R) == 8) sincos(2.0 * kPi * ua, &sn, &cs);
+            if constexpr (sizeof(R) == 8) sincos_turns(ua - rint(ua), sn, cs);
             else sincos_turns(float(ua - rint(ua)), sn, cs);
             const R rad = sqrt_r<R>(R(-2) * log_r<R>(R(ub)));
             const R gx = sn * rad, gy = cs * rad;
@@ -595,9 +680,9 @@ __device__ __forceinline__ void sm_moves_walker(const DevSys &S, double *sm, int
                     const double drn = fmin(dcut, fmax(-dcut, dn[d] * sig2));
                     lq += (dro[d] + drn) * (dro[d] - drn - 2.0 * dxm[d]);
                 }
-                const double logp = 2.0 * (log(fabs(rho)) + dU) + lq * half_isig2;
+                const double logp = 2.0 * (log_r<double>(fabs(rho)) + dU) + lq * half_isig2;
                 logp_rec = logp;
-                accept = u4 < exp(fmin(0.0, logp));
+                accept = u4 < exp_r<double>(fmin(0.0, logp));
             } else {
                 const float fs2 = float(sig2), fcut = float(dcut);
                 float lq = 0.0f;
