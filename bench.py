@@ -196,7 +196,12 @@ def precision_study(nconf=4096, ks=(N, 33, 16), nequil=200):
 
 
 def _sample_energy(e, nequil, nblocks, blocklen):
-    """block means of the weighted energy estimator E_L / (N E_FG) after nequil steps"""
+    """block means of the weighted energy estimator E_L / (N E_FG) after nequil steps: their
+    mean and standard error, and the number of blocks left out as outliers.  A block is an
+    outlier when its mean lies more than 10 robust standard deviations (1.4826 times the
+    median absolute deviation) from the median block: one walker whose local energy spikes
+    (a pair landing next to a node or in the narrow potential well) carries an uncapped
+    estimator weight exp(-(E_L - E_T) dt) (step.f90:137-156) that decides its block alone."""
     for _ in range(nequil):
         e.step()
     e.estimators(reset=True)
@@ -207,7 +212,11 @@ def _sample_energy(e, nequil, nblocks, blocklen):
         est = e.estimators(reset=True)
         blocks.append(est[0, 0] / est[0, 2])
     b = np.array(blocks)
-    return float(b.mean()), float(b.std(ddof=1) / np.sqrt(nblocks))
+    med = np.median(b[np.isfinite(b)]) if np.isfinite(b).any() else 0.0
+    mad = 1.4826 * np.median(np.abs(b[np.isfinite(b)] - med)) if np.isfinite(b).any() else 0.0
+    keep = np.isfinite(b) & (np.abs(b - med) <= 10.0 * max(mad, 1e-12))
+    k = b[keep]
+    return (float(k.mean()), float(k.std(ddof=1) / np.sqrt(len(k))), int((~keep).sum()))
 
 
 def energy_study(nequil=2000, nblocks=20, blocklen=50):
@@ -257,7 +266,7 @@ def energy_study(nequil=2000, nblocks=20, blocklen=50):
     return dict(unit="E_L / (N E_FG)", tau=DT,
                 start=f"{nequil} mixed steps from the reference's initial walkers",
                 blocks=f"{nblocks} blocks of {blocklen} steps at tau, {2 * blocklen} at tau/2",
-                runs={k: dict(e=v[0], err=v[1]) for k, v in runs.items()},
+                runs={k: dict(e=v[0], err=v[1], outlier_blocks=v[2]) for k, v in runs.items()},
                 mixed_vs_fp64=diff(runs["mixed"], runs["fp64"]),
                 singleparticle_vs_allparticle_tau=diff(runs["fp64"], runs["allparticle"]),
                 singleparticle_vs_allparticle_half_tau=diff(runs["fp64_half_tau"],
