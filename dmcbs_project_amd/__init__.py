@@ -6,9 +6,9 @@ mirror the reference's Fortran module interfaces (step, stack, wavefunction).  T
 ``observables`` (g(r), density profile, S(q), n(k): gofr, sofq, pdistribution) and ``deck``
 (the positional input deck and the walker file fort.9) complete the host side.
 """
-from .lib import (DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC, DMCError,
-                  load)
+from .lib import (DMC_FP64, DMC_MIXED, DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC,
+                  DMC_METROPOLIS, DMCError, load)
 from .api import Params, Engine, Batch
 
-__all__ = ["Params", "Engine", "Batch", "DMCError", "load", "DMC_FP64", "DMC_MIXED", "DMC_ALLPARTICLE",
-           "DMC_SINGLEPARTICLE", "DMC_VMC"]
+__all__ = ["Params", "Engine", "Batch", "DMCError", "load", "DMC_FP64", "DMC_MIXED",
+           "DMC_ALLPARTICLE", "DMC_SINGLEPARTICLE", "DMC_VMC", "DMC_METROPOLIS"]

@@ -204,6 +204,141 @@ __global__ __launch_bounds__(64, 2) void k_vmc(DevSys S, DevState A, DevState B,
     if (tid < 42) T.est[size_t(w) * 42 + tid] = est[tid];
 }
 
+// The Metropolis DMC step of the reference (step1 with idmc = 2, step.f90:56-76,161-226): the
+// all-particle drift-diffusion proposal and Metropolis test of k_vmc, then the weight of the
+// state kept: exp(-((E_old + E_new)/2 - E_T) dt), capped, for an accepted move (zero after a
+// node crossing under the fixed-node rule), exp(-(E_old - E_T) dt) times the walker's weight,
+// capped, for a rejected one; the 13 estimators of idmc = 2 (dmcbcs.f90:204-218: energies,
+// node crossing, acceptance, |psit|/psig, psit, psig, sign, wt, wtg, sign(wtg)) and the
+// branch draw (step.f90:302-305) as in k_step.
+template <int NUP>
+__global__ __launch_bounds__(64, 2) void k_mdmc(DevSys S, DevState A, DevState B, DevStep T,
+                                             int opcomp) {
+    extern __shared__ double sm[];
+    const Layout<NUP> L(S.nq, true);
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= T.nt[0]) return;
+    constexpr int M = 6 * NUP;
+    constexpr int ND = 2 * ((M + 1) / 2);
+    load_q<NUP>(S, sm, L, tid);
+    const uint64_t s0 = A.irn[w];
+    const double sig = S.sigma, dfac = S.sig2 * S.sqrtm;
+    const double *xo = A.x + size_t(w) * M, *go = A.dpsi + size_t(w) * M;
+    for (int m = tid; 2 * m < M; m += 64) {
+        double g[2];
+        gauss_pair(S, s0, m, M, g[0], g[1]);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int t = 2 * m + e;
+            if (t < M) {
+                double drift = go[t] * dfac;
+                if (S.cutdrift) drift = fmin(S.dcut, fmax(-S.dcut, drift));
+                sm[L.xs + t] = wrapd(wrapd(xo[t], S) + (drift + sig * g[e]), S);
+            }
+        }
+    }
+    wave_sync();
+    hpsi_block<NUP>(S, sm, L, tid);
+    double gterm = 0.0;
+    for (int m = tid; 2 * m < M; m += 64) {
+        double g[2];
+        gauss_pair(S, s0, m, M, g[0], g[1]);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int t = 2 * m + e;
+            if (t < M) {
+                double dro = go[t] * dfac, drn = sm[L.dp + t] * dfac;
+                if (S.cutdrift) {
+                    dro = fmin(S.dcut, fmax(-S.dcut, dro));
+                    drn = fmin(S.dcut, fmax(-S.dcut, drn));
+                }
+                const double dx = dro + sig * g[e];
+                gterm += (dro + drn) * (dro - drn - 2.0 * dx);
+            }
+        }
+    }
+    gterm = wave_sum(gterm);
+    const double psil2 = sm[L.scal], d2psi2 = sm[L.scal + 1], v2 = sm[L.scal + 2];
+    const int is2 = int(sm[L.scal + 3]);
+    const double prob = exp(fmin(0.0, 2.0 * (psil2 - A.psil[w]) + gterm / (2.0 * S.sig2)));
+    uint64_t s1 = (S.ja1[ND - 1] * s0 + S.jc1[ND - 1]) & kMask48;  // after gaussian(M)
+    s1 = (s1 * kM1 + kA1) & kMask48;                                 // randn(1)
+    const bool acc = double(s1 | 1ull) * 0x1p-48 < prob;
+    double *xn = B.x + size_t(w) * M, *gn = B.dpsi + size_t(w) * M;
+    for (int t = tid; t < M; t += 64) {
+        xn[t] = acc ? sm[L.xs + t] : wrapd(xo[t], S);
+        gn[t] = acc ? sm[L.dp + t] : go[t];
+    }
+    double *est = sm + L.est;
+    if (tid < 42) est[tid] = 0.0;
+    wave_sync();
+    if (tid == 0) {
+        auto put = [&](int e, double val, double wgt) {
+            est[3 * e] = val * wgt;
+            est[3 * e + 1] = val * val * wgt;
+            est[3 * e + 2] = wgt;
+        };
+        const double eo = -S.hbar * A.d2psi[w] + A.v[w], nf = S.nfac, wt0 = A.wt0[w];
+        double wt, weight, e, v, psil;
+        int is;
+        if (acc) {
+            e = -S.hbar * d2psi2 + v2;
+            v = v2;
+            psil = psil2;
+            is = is2;
+            wt = exp(-(0.5 * (eo + e) - S.etrial) * S.dt);
+            weight = fmin(S.wtcut, wt);
+            const bool cross = is2 != A.is[w];
+            if (cross && S.fn) {
+                weight = 0.0;
+                wt = 0.0;
+            }
+            put(3, cross ? 1.0 : 0.0, 1.0);  // node crossing, accepted moves only
+        } else {
+            e = eo;
+            v = A.v[w];
+            psil = A.psil[w];
+            is = A.is[w];
+            wt = exp(-(eo - S.etrial) * S.dt);
+            weight = fmin(S.wtcut, wt * A.weight[w]);
+        }
+        put(4, acc ? 1.0 : 0.0, 1.0);  // acceptance
+        const double wtg = wt * double(is) * wt0;
+        if (opcomp) {
+            const double sg = copysign(1.0, wtg);
+            put(0, e * nf, wtg);
+            put(1, (e - v) * nf, wtg);
+            put(2, v * nf, wtg);
+            put(5, 1.0, 1.0);
+            put(6, is * exp(psil), 1.0);
+            put(7, exp(psil), 1.0);
+            put(8, double(is), 1.0);
+            put(9, wt, 1.0);
+            put(10, wtg, 1.0);
+            put(11, sg, 1.0);
+            put(12, sg, wtg);
+        }
+        s1 = (s1 * kM1 + kA1) & kMask48;  // the branch draw
+        const double ub = double(s1 | 1ull) * 0x1p-48;
+        const int iwt = int(weight + ub);
+        T.count[w] = iwt;
+        if (iwt >= 0 && iwt < 500)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.hist[iwt]), 1ull);
+        B.irn[w] = s1;
+        B.psil[w] = psil;
+        B.d2psi[w] = acc ? d2psi2 : A.d2psi[w];
+        B.v[w] = v;
+        B.is[w] = is;
+        B.wt0[w] = wt0;
+        B.weight[w] = weight;
+        T.nacc[w] = acc ? 1 : 0;
+        if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[0]), 1ull);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[1]), 1ull);
+    }
+    wave_sync();
+    if (tid < 42) T.est[size_t(w) * 42 + tid] = est[tid];
+}
+
 // single-particle ratio psi(x')/psi(x) (move1, wavefunction.f90:393-404): determinant
 // ratio from the inverse (bcsone, psislaterbcs.f90:296-376) times the Jastrow ratio
 template <int NUP>
@@ -289,6 +424,14 @@ int launch_vmc(int nup, const DevSys &S, const DevState &A, const DevState &B, c
     if (grid <= 0) return hipSuccess;
     DMC_DISPATCH(nup, hipLaunchKernelGGL(k_vmc<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq, true), st,
                                          S, A, B, T));
+    return hipGetLastError();
+}
+
+int launch_mdmc(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                const DevStep &T, int opcomp, int grid, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
+    DMC_DISPATCH(nup, hipLaunchKernelGGL(k_mdmc<N_>, dim3(grid), dim3(64), lds_bytes<N_>(S.nq, true), st,
+                                         S, A, B, T, opcomp));
     return hipGetLastError();
 }
 

@@ -397,9 +397,10 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
         return fail(DMC_ERR_ARG, "precision must be DMC_FP64 or DMC_MIXED");
     }
     if (p->movetype != DMC_ALLPARTICLE && p->movetype != DMC_SINGLEPARTICLE &&
-        p->movetype != DMC_VMC) {
+        p->movetype != DMC_VMC && p->movetype != DMC_METROPOLIS) {
         delete c;
-        return fail(DMC_ERR_ARG, "movetype must be DMC_ALLPARTICLE, DMC_SINGLEPARTICLE or DMC_VMC");
+        return fail(DMC_ERR_ARG, "movetype must be DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC "
+                                 "or DMC_METROPOLIS");
     }
     if (p->krefresh < 0) {
         delete c;
@@ -407,7 +408,7 @@ int dmc_create(const dmc_params *p, dmc_ctx **out) {
     }
     if (p->movetype != DMC_SINGLEPARTICLE && p->precision != DMC_FP64) {
         delete c;
-        return fail(DMC_ERR_ARG, "all-particle and VMC moves run in FP64 only");
+        return fail(DMC_ERR_ARG, "all-particle, Metropolis and VMC moves run in FP64 only");
     }
     c->capacity = p->capacity > 0 ? p->capacity : 4 * (p->nwalk > 0 ? p->nwalk : 1);
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
@@ -691,6 +692,11 @@ int dmc_propagate(dmc_ctx *c, int32_t opcomp) {
         LAUNCHCHK(launch_est_reduce(c->T, n, c->est_acc, c->st));
         std::swap(c->A, c->B);
         return DMC_OK;
+    } else if (c->sys.p.movetype == DMC_METROPOLIS) {
+        if (c->timing) HIPCHK(hipEventRecord(c->evs[0], c->st));
+        LAUNCHCHK(launch_mdmc(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));
+        if (c->timing) HIPCHK(hipEventRecord(c->evs[1], c->st));
+        nev = c->timing ? 1 : 0;
     } else {
         if (c->timing) HIPCHK(hipEventRecord(c->evs[0], c->st));
         LAUNCHCHK(launch_step(c->sys.n, c->ds, c->A, c->B, c->T, opcomp, n, c->st));

@@ -19,6 +19,9 @@ int launch_step(int nup, const DevSys &S, const DevState &A, const DevState &B,
 // variational (Metropolis) step of state A into state B, no branching
 int launch_vmc(int nup, const DevSys &S, const DevState &A, const DevState &B, const DevStep &T,
                int grid, hipStream_t st);
+// Metropolis DMC step (step1 with idmc = 2) of state A into state B with the branch draw
+int launch_mdmc(int nup, const DevSys &S, const DevState &A, const DevState &B,
+                const DevStep &T, int opcomp, int grid, hipStream_t st);
 // single-particle moves m0..m1-1 of every walker, from positions/RNG xin/irn_in and the
 // stored inverse sinv, to xout/irn_out (move kernel)
 int launch_sm_moves(int nup, const DevSys &S, const double *xin, double *xout,

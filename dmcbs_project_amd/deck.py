@@ -135,8 +135,11 @@ def read_deck(text, **device):
     if run["idmc"] in (0, -1):
         # variational runs (step1 with idmc = 0, step.f90:78-122), as the Fortran host maps them
         device = dict(device, movetype=L.DMC_VMC, precision=L.DMC_FP64)
+    elif run["idmc"] == 2:
+        # Metropolis DMC (step1 with idmc = 2, step.f90:161-226)
+        device = dict(device, movetype=L.DMC_METROPOLIS, precision=L.DMC_FP64)
     elif run["idmc"] != 1:
-        raise ValueError("the engine runs DMC (idmc = 1) and VMC (idmc = 0, -1)")
+        raise ValueError("the engine runs DMC (idmc = 1, 2) and VMC (idmc = 0, -1)")
     p = Params(npart=npart, norb=norb, ntab=ntab, fn=fn, units=units, nwalk=nwalk, rho=rho,
                hbar=hbar, mass=mass[0], dt=dt, etrial=etrial, dcut=dcut, wtcut=wtcut,
                popup=popup, popdn=popdn, bbeta=bbeta, dbeta=dbeta, bfact=bfact, v0=v0,

@@ -6,7 +6,8 @@ module dmc_gpu
    integer(c_int), parameter :: DMC_OK=0, DMC_ERR_ARG=1, DMC_ERR_HIP=2, DMC_ERR_OVERFLOW=3, &
       DMC_ERR_SETUP=4, DMC_ERR_NODEVICE=5
    integer(c_int32_t), parameter :: DMC_FP64=0, DMC_MIXED=1
-   integer(c_int32_t), parameter :: DMC_ALLPARTICLE=0, DMC_SINGLEPARTICLE=1, DMC_VMC=2
+   integer(c_int32_t), parameter :: DMC_ALLPARTICLE=0, DMC_SINGLEPARTICLE=1, DMC_VMC=2, &
+      DMC_METROPOLIS=3
 
    type, bind(c) :: dmc_params
       integer(c_int32_t) :: npart, norb, ntab, fn, units, nwalk, capacity, precision, &

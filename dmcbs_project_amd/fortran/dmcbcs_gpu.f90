@@ -6,8 +6,9 @@
 !
 ! Usage:  dmcbcs_gpu [fp64|mixed] [allparticle|singleparticle] [krefresh] < deck
 ! The defaults (fp64 allparticle) propagate exactly as the reference's step1.  Decks with
-! idmc = 0 run variational Monte Carlo, idmc = -1 adds the parameter optimisation
-! (calder after every block, updatepar after each of optiter iterations).
+! idmc = 2 run Metropolis DMC, idmc = 0 variational Monte Carlo, and idmc = -1 adds the
+! parameter optimisation (calder after every block, updatepar after each of optiter
+! iterations).
 program dmcbcs_gpu
    use engine
    use potential
@@ -83,6 +84,10 @@ program dmcbcs_gpu
       prec=DMC_FP64
       movet=DMC_VMC
    endif
+   if (idmc.eq.2) then
+      prec=DMC_FP64
+      movet=DMC_METROPOLIS
+   endif
    call engine_deck(nwalk,etrial0,units,popup,popdn,prec,movet,kref,int(10*nwalk,i4))
 
 ! module set-up in the order of dmcbcs.f90:142-177
@@ -119,6 +124,23 @@ program dmcbcs_gpu
       call addest(4,'external potential')
       call addest(5,'jackson-feenberg energy')
       call addest(6,'acceptance =')
+   elseif (idmc.eq.2) then
+! the Metropolis DMC estimators of dmcbcs.f90:204-218
+      call setestnum(13)
+      allocate(answer(13))
+      call addest(1,'total energy')
+      call addest(2,'kinetic energy')
+      call addest(3,'potential energy')
+      call addest(4,'node crossing')
+      call addest(5,'acceptance')
+      call addest(6,'|psit|/psig =')
+      call addest(7,'psit =')
+      call addest(8,'psig =')
+      call addest(9,'sign(psit) =')
+      call addest(10,'wt =')
+      call addest(11,'wtg =')
+      call addest(12,'sign(wtg) =')
+      call addest(13,'sign(wtg)-weighted =')
    else
    call setestnum(14)
    allocate(answer(14))

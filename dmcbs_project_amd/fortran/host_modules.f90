@@ -461,8 +461,8 @@ contains
    real(kind=r8) :: hbar,dt,etrial,el,mass(:),wtcut,dcut,eunit
    logical :: fn
    idmc=idmcin
-   if (idmc.ne.1.and.idmc.ne.0.and.idmc.ne.-1) &
-      error stop 'setstep: the engine runs DMC (idmc = 1) and VMC (idmc = 0, -1)'
+   if (idmc.ne.1.and.idmc.ne.2.and.idmc.ne.0.and.idmc.ne.-1) &
+      error stop 'setstep: the engine runs DMC (idmc = 1, 2) and VMC (idmc = 0, -1)'
    prm%hbar=hbar
    prm%dt=dt
    prm%wtcut=wtcut

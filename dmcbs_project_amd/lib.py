@@ -13,7 +13,7 @@ LIBPATH = os.path.join(HERE, "libdmc_gpu.so")
 MAXORB = 16
 
 DMC_FP64, DMC_MIXED = 0, 1
-DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC = 0, 1, 2
+DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC, DMC_METROPOLIS = 0, 1, 2, 3
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "population overflow",
           4: "set-up failure", 5: "no HIP device"}
 

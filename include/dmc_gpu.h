@@ -35,10 +35,13 @@ enum {
 enum { DMC_FP64 = 0, DMC_MIXED = 1 };
 /* move type: the reference's all-particle DMC move (step.f90:56-76), single-particle DMC
  * moves with Sherman-Morrison inverse updates and an FP64 rebuild of the inverse (Gauss-Jordan
- * elimination with partial pivoting) every krefresh moves, or
+ * elimination with partial pivoting) every krefresh moves,
  * the reference's variational Metropolis step (step1 with idmc = 0, step.f90:78-122: no
- * weights, no branching; estimators 1-6 every step) */
-enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1, DMC_VMC = 2 };
+ * weights, no branching; estimators 1-6 every step), or the reference's Metropolis DMC step
+ * (step1 with idmc = 2, step.f90:161-226: the all-particle proposal accepted or rejected by
+ * the Metropolis test, then weighted and branched; estimators 1-13 in the order of
+ * dmcbcs.f90:204-218) */
+enum { DMC_ALLPARTICLE = 0, DMC_SINGLEPARTICLE = 1, DMC_VMC = 2, DMC_METROPOLIS = 3 };
 
 /* Run parameters: the reference's deck (dmcbcs.f90:43-66, psislaterbcs.f90:47-68,
  * betafun.f90:19-21, coshpot.f90:35-41) plus the device options. */
@@ -51,7 +54,7 @@ typedef struct dmc_params {
     int32_t nwalk;      /* target population */
     int32_t capacity;   /* device walker capacity (0: 4 * nwalk) */
     int32_t precision;  /* DMC_FP64 or DMC_MIXED */
-    int32_t movetype;   /* DMC_ALLPARTICLE or DMC_SINGLEPARTICLE */
+    int32_t movetype;   /* DMC_ALLPARTICLE, DMC_SINGLEPARTICLE, DMC_VMC or DMC_METROPOLIS */
     int32_t krefresh;   /* single-particle moves between FP64 LU refreshes (0: npart) */
     double rho, hbar, mass, dt, etrial, dcut, wtcut, popup, popdn;
     double bbeta, dbeta, bfact, v0, amu, cjas1, cjas2, djas, v0op, v0eq;

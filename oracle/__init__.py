@@ -93,6 +93,8 @@ def _bind(L):
                                 C.c_int32, _f64p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
                                 _i32p, C.c_int32]
     L.dmco_dmc_step.restype = C.c_int
+    L.dmco_mdmc_step.argtypes = L.dmco_dmc_step.argtypes
+    L.dmco_mdmc_step.restype = C.c_int
     L.dmco_sm_step1.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p, _i64p,
                                 C.c_int32, _f64p, _f64p, _f64p, _i64p, _f64p, _i32p,
                                 C.c_int32]
@@ -326,7 +328,9 @@ class System:
                               est, xo.reshape(-1), do.reshape(-1), so.reshape(-1), ro, nthreads)
         return xo, do, so, ro, est
 
-    def dmc_step(self, x, dpsi, sc, irn, opcomp=False, est=None, cap=None, nthreads=0):
+    def dmc_step(self, x, dpsi, sc, irn, opcomp=False, est=None, cap=None, nthreads=0,
+                 metropolis=False):
+        """step1 + branch + population control; metropolis: step1 with idmc = 2"""
         nw = len(x)
         cap = cap or max(4 * nw, 64)
         est = np.zeros(42) if est is None else est
@@ -335,7 +339,8 @@ class System:
         so = np.zeros((cap, 6))
         ro = np.zeros(cap, np.int64)
         nt = np.zeros(2, np.int32)
-        rc = self.L.dmco_dmc_step(self.h, nw, np.ascontiguousarray(x).reshape(-1),
+        fn = self.L.dmco_mdmc_step if metropolis else self.L.dmco_dmc_step
+        rc = fn(self.h, nw, np.ascontiguousarray(x).reshape(-1),
                                   np.ascontiguousarray(dpsi).reshape(-1),
                                   np.ascontiguousarray(sc).reshape(-1), np.ascontiguousarray(irn),
                                   int(opcomp), est, cap, xo.reshape(-1), do.reshape(-1),

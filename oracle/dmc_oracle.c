@@ -1017,6 +1017,117 @@ void dmco_vmc_step1(const dmco_ctx *c, int32_t nw, const double *x, const double
     free(pe);
 }
 
+/* step.f90:56-76,161-226 step1 with idmc = 2 (Metropolis DMC) for one walker: the proposal
+ * and the Metropolis test of vmc_one; an accepted walker is weighted exp(-((E_old + E_new)/2 -
+ * E_T) dt), capped, with the node test, a rejected one exp(-(E_old - E_T) dt) times its weight,
+ * capped; estimators 1-13 of dmcbcs.f90:204-218 (node crossing 4 on accepted moves only,
+ * acceptance 5 always, the rest on opcomp). */
+static void mdmc_one(const dmco_ctx *c, const double *x, const double *dpsi, const double *sc,
+                     int64_t irn, int opcomp, double *est, double *xo, double *dpsio,
+                     double *sco, int64_t *irno, double *wto) {
+    int np = c->npart;
+    double hbar = c->p.hbar, el = c->el, nfac = c->nfac;
+    double *g = malloc(sizeof(double) * 3 * np), *x1 = malloc(sizeof(double) * 3 * np);
+    double *dro = malloc(sizeof(double) * 3 * np), *dx = malloc(sizeof(double) * 3 * np);
+    double *xn = malloc(sizeof(double) * 3 * np), *dn = malloc(sizeof(double) * 3 * np);
+    for (int i = 0; i < 3 * np; i++) x1[i] = wrapd(x[i], el);
+    int64_t s = irn;
+    dmco_gaussian(&s, 3 * np, g);
+    double sig = c->sigma, sm = sqrt(c->p.mass), dc = sig * c->p.dcut;
+    for (int i = 0; i < 3 * np; i++) {
+        double drift = dpsi[i] * (sig * sig) * sm;
+        if (c->p.dcut >= 0.0) drift = fmin(dc, fmax(-dc, drift));
+        dro[i] = drift;
+        dx[i] = drift + sig * g[i];
+        xn[i] = wrapd(x1[i] + dx[i], el);
+    }
+    double o[4];
+    dmco_hpsi(c, xn, o, dn);
+    double prob = 2.0 * (o[0] - sc[1]);
+    for (int i = 0; i < np; i++) {
+        double dot = 0.0;
+        for (int d = 0; d < 3; d++) {
+            int t = 3 * i + d;
+            double drn = dn[t] * (sig * sig) * sm;
+            if (c->p.dcut >= 0.0) drn = fmin(dc, fmax(-dc, drn));
+            dot += (dro[t] + drn) * (dro[t] - drn - 2.0 * dx[t]);
+        }
+        prob += dot / (2.0 * sig * sig);
+    }
+    prob = exp(fmin(0.0, prob));
+    double rn = dmco_ran1(&s);
+    int acc = rn < prob;
+    double eo = -hbar * sc[2] + sc[3], wt0 = sc[5], wt, weight, e, v, psil;
+    int is;
+    if (acc) {
+        e = -hbar * o[1] + o[2];
+        v = o[2];
+        psil = o[0];
+        is = (int)o[3];
+        wt = exp(-(0.5 * (eo + e) - c->etrial) * c->p.dt);
+        weight = fmin(c->p.wtcut, wt);
+        if (is != (int)sc[4]) {
+            if (c->p.fn) { weight = 0.0; wt = 0.0; }
+            addval(est, 4, 1.0, 1.0);
+        } else {
+            addval(est, 4, 0.0, 1.0);
+        }
+    } else {
+        e = eo;
+        v = sc[3];
+        psil = sc[1];
+        is = (int)sc[4];
+        wt = exp(-(eo - c->etrial) * c->p.dt);
+        weight = fmin(c->p.wtcut, wt * sc[0]);
+    }
+    double wtg = wt * is * wt0;
+    if (opcomp) {
+        addval(est, 1, e * nfac, wtg);
+        addval(est, 2, (e - v) * nfac, wtg);
+        addval(est, 3, v * nfac, wtg);
+        addval(est, 6, 1.0, 1.0);
+        addval(est, 7, is * exp(psil), 1.0);
+        addval(est, 8, exp(psil), 1.0);
+        addval(est, 9, 1.0 * is, 1.0);
+        addval(est, 10, wt, 1.0);
+        addval(est, 11, wtg, 1.0);
+        addval(est, 12, copysign(1.0, wtg), 1.0);
+        addval(est, 13, copysign(1.0, wtg), wtg);
+    }
+    addval(est, 5, acc ? 1.0 : 0.0, 1.0);
+    memcpy(xo, acc ? xn : x1, sizeof(double) * 3 * np);
+    memcpy(dpsio, acc ? dn : dpsi, sizeof(double) * 3 * np);
+    sco[0] = weight;
+    sco[1] = psil;
+    sco[2] = acc ? o[1] : sc[2];
+    sco[3] = v;
+    sco[4] = is;
+    sco[5] = wt0;
+    *irno = s;
+    if (wto) *wto = wt;
+    free(g); free(x1); free(dro); free(dx); free(xn); free(dn);
+}
+
+void dmco_mdmc_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                     const double *sc, const int64_t *irn, int32_t opcomp, double *est,
+                     double *x_out, double *dpsi_out, double *sc_out, int64_t *irn_out,
+                     double *wt_out, int32_t nthreads) {
+    int np = c->npart;
+    double *pe = calloc((size_t)nw * 42, sizeof(double));
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int w = 0; w < nw; w++) {
+        size_t o = (size_t)3 * np * w;
+        mdmc_one(c, x + o, dpsi + o, sc + 6 * w, irn[w], opcomp, pe + 42 * w, x_out + o,
+                 dpsi_out + o, sc_out + 6 * w, irn_out + w, wt_out ? wt_out + w : NULL);
+    }
+    for (int w = 0; w < nw; w++)
+        for (int k = 0; k < 42; k++) est[k] += pe[42 * w + k];
+    free(pe);
+}
+
 /* step.f90:287-315 branch */
 int dmco_branch(int32_t nw, int32_t npart, const double *x, const double *dpsi,
                 const double *sc, const int64_t *irn, double factor, int32_t cap,
@@ -1042,17 +1153,44 @@ int dmco_branch(int32_t nw, int32_t npart, const double *x, const double *dpsi,
     return nt;
 }
 
-/* dmcbcs.f90:437-445 */
+typedef void (*step1_fn)(const dmco_ctx *, int32_t, const double *, const double *,
+                         const double *, const int64_t *, int32_t, double *, double *, double *,
+                         double *, int64_t *, double *, int32_t);
+
+/* dmcbcs.f90:437-445 with the step1 of idmc = 1 (dmco_dmc_step) or idmc = 2 (dmco_mdmc_step) */
+static int dmc_step_with(step1_fn step1, const dmco_ctx *c, int32_t nw, const double *x,
+                         const double *dpsi, const double *sc, const int64_t *irn,
+                         int32_t opcomp, double *est, int32_t cap, double *x_out,
+                         double *dpsi_out, double *sc_out, int64_t *irn_out, int32_t *nt,
+                         int32_t nthreads);
+
 int dmco_dmc_step(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
                   const double *sc, const int64_t *irn, int32_t opcomp, double *est,
                   int32_t cap, double *x_out, double *dpsi_out, double *sc_out,
                   int64_t *irn_out, int32_t *nt, int32_t nthreads) {
+    return dmc_step_with(dmco_step1, c, nw, x, dpsi, sc, irn, opcomp, est, cap, x_out,
+                         dpsi_out, sc_out, irn_out, nt, nthreads);
+}
+
+int dmco_mdmc_step(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                   const double *sc, const int64_t *irn, int32_t opcomp, double *est,
+                   int32_t cap, double *x_out, double *dpsi_out, double *sc_out,
+                   int64_t *irn_out, int32_t *nt, int32_t nthreads) {
+    return dmc_step_with(dmco_mdmc_step1, c, nw, x, dpsi, sc, irn, opcomp, est, cap, x_out,
+                         dpsi_out, sc_out, irn_out, nt, nthreads);
+}
+
+static int dmc_step_with(step1_fn step1, const dmco_ctx *c, int32_t nw, const double *x,
+                         const double *dpsi, const double *sc, const int64_t *irn,
+                         int32_t opcomp, double *est, int32_t cap, double *x_out,
+                         double *dpsi_out, double *sc_out, int64_t *irn_out, int32_t *nt,
+                         int32_t nthreads) {
     int np = c->npart;
     size_t m = (size_t)3 * np;
     double *x1 = malloc(sizeof(double) * m * nw), *d1 = malloc(sizeof(double) * m * nw);
     double *s1 = malloc(sizeof(double) * 6 * nw);
     int64_t *r1 = malloc(sizeof(int64_t) * nw);
-    dmco_step1(c, nw, x, dpsi, sc, irn, opcomp, est, x1, d1, s1, r1, NULL, nthreads);
+    step1(c, nw, x, dpsi, sc, irn, opcomp, est, x1, d1, s1, r1, NULL, nthreads);
     int n1 = dmco_branch(nw, np, x1, d1, s1, r1, 1.0, cap, x_out, dpsi_out, sc_out, irn_out);
     free(x1); free(d1); free(s1); free(r1);
     if (n1 < 0) return -1;

@@ -78,6 +78,16 @@ int dmco_dmc_step(const dmco_ctx *c, int32_t nw, const double *x, const double *
                   const double *sc, const int64_t *irn, int32_t opcomp, double *est,
                   int32_t cap, double *x_out, double *dpsi_out, double *sc_out,
                   int64_t *irn_out, int32_t *nt, int32_t nthreads);
+/* the same with step1 of idmc = 2, Metropolis DMC (step.f90:161-226); est holds
+ * estimators 1-13 of dmcbcs.f90:204-218 */
+int dmco_mdmc_step(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                   const double *sc, const int64_t *irn, int32_t opcomp, double *est,
+                   int32_t cap, double *x_out, double *dpsi_out, double *sc_out,
+                   int64_t *irn_out, int32_t *nt, int32_t nthreads);
+void dmco_mdmc_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,
+                     const double *sc, const int64_t *irn, int32_t opcomp, double *est,
+                     double *x_out, double *dpsi_out, double *sc_out, int64_t *irn_out,
+                     double *wt_out, int32_t nthreads);
 /* the step1 part alone for every walker (no branching): new state + per-walker
  * uncapped weight wt (0 after a node crossing with fn) */
 void dmco_step1(const dmco_ctx *c, int32_t nw, const double *x, const double *dpsi,

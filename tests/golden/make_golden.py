@@ -276,6 +276,29 @@ def vmc_long_fixture(n=2, nwalk=512, dt=0.05, dcut=1.0, first=500, last=3000, ev
                         elocal=np.array(el, np.float32), r12=np.array(r12, np.float32), deck=deck)
 
 
+def main_mdmc():
+    """Metropolis DMC (step1 with idmc = 2, step.f90:161-226): short trajectories with walker
+    dumps at N = 14 and 66.  With this deck the branch grows the population step by step
+    (walkers whose proposals keep being rejected are weighted exp(-(E_old - E_T) dt) again
+    and again); at dt = 0.002 the reference stops with a stack overflow in its sixth step,
+    whose population trace up to the stop is kept as well."""
+    dmc_fixture("mdmc_N14_short", [0, 1, 2, 5, 10, 20], idmc=2, npart=14, norb=3, nwalk=200,
+                nav=2, nstep=10, dt=0.0005, seed=1)
+    dmc_fixture("mdmc_N66_short", [0, 1, 2, 4], idmc=2, npart=66, norb=5, nwalk=32, nav=2,
+                nstep=2, dt=0.001, seed=7)
+    deck, p = refdeck.make_deck(idmc=2, npart=14, norb=3, nwalk=200, nav=2, nstep=10,
+                                dt=0.002, seed=1)
+    data, out = refrun.run("dmc", deck, pack([0], np.zeros(0, np.int64)))
+    assert "stack overflow" in out
+    r = Reader(data)
+    nsteps = r.i()
+    pops = []
+    while not r.done():
+        pops.append(r.i(2))
+    np.savez(os.path.join(HERE, "mdmc_N14_overflow.npz"), npart=14, norb=3, nwalk=200,
+             dt=0.002, seed=1, nsteps=nsteps, pop=np.array(pops), capacity=2000, deck=deck)
+
+
 def _table(text):
     """rows of numbers of one output table (comment and blank lines skipped)"""
     rows = []
@@ -373,6 +396,8 @@ if __name__ == "__main__":
         main_vmc()
     elif "--vmclong" in sys.argv:
         vmc_long_fixture()
+    elif "--mdmc" in sys.argv:
+        main_mdmc()
     elif "--obs" in sys.argv:
         main_obs()
     elif "--enk" in sys.argv:

@@ -20,15 +20,15 @@ def run(deck, cwd, *args):
     return r.stdout
 
 
-def parse(out):
-    """Per block: walker number and the 14 (value, error) pairs of resstring."""
+def parse(out, nest=14):
+    """Per block: walker number and the nest (value, error) pairs of resstring."""
     blocks = []
     lines = out.splitlines()
     for k, line in enumerate(lines):
         if line.startswith("walker number"):
             nt = int(line.split()[-1])
             est = []
-            for e in lines[k + 1:k + 15]:
+            for e in lines[k + 1:k + 1 + nest]:
                 tok = e[30:].split()
                 est.append((float(tok[1]), float(tok[3])))
             blocks.append((nt, np.array(est)))
@@ -192,3 +192,16 @@ def test_fragment_hpsi_pass_calder_and_walker_file(tmp_path, oracle_mod):
     xf, wf, pl, pg, irnf, tau = deckmod.read_walkers(str(tmp_path / "fort.9"), 14)
     np.testing.assert_array_equal(np.sort(irnf), np.sort(irn))
     np.testing.assert_array_equal(pl, pg)
+
+
+def test_driver_metropolis_dmc_blocks(tmp_path):
+    """A reference deck with idmc = 2 (Metropolis DMC) through the driver: the walker
+    numbers and the 13 block estimators of the reference's run."""
+    g = golden("dmc_mdmc_N14_short.npz")
+    blocks = parse(run(str(g["deck"]), tmp_path), nest=13)
+    nstep = int(g["nstep"])
+    assert len(blocks) == int(g["nav"])
+    for b, (nt, est) in enumerate(blocks):
+        assert nt == int(g["pop"][(b + 1) * nstep - 1][1])
+        for i in (0, 1, 4, 9):  # total, kinetic energy, acceptance, wt
+            np.testing.assert_allclose(est[i, 0], g["est"][b][i, 0], rtol=1e-8, atol=1e-12)

@@ -210,3 +210,53 @@ def test_optimisation_run(oracle_mod):
     ene, par = oracle_mod.calder_updatepar(s, np.concatenate(confs), 0.7, 1e-4)
     assert ene == pytest.approx(float(g["opt_ene"]), rel=1e-10)
     np.testing.assert_allclose(par, g["opt_par"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("tag", ["mdmc_N14_short", "mdmc_N66_short"])
+def test_metropolis_dmc_trajectory(oracle_mod, tag):
+    """Metropolis DMC (step1 with idmc = 2, step.f90:161-226): walker multisets after every
+    dumped step, the population trace, and the 13 block estimators of dmcbcs.f90:204-218
+    against the reference's run."""
+    g = golden(f"dmc_{tag}.npz")
+    s, x, dpsi, sc, irn = _run_reference_trajectory(oracle_mod, g)
+    dumps = set(int(d) for d in g["dumps"])
+    nstep = int(g["nstep"])
+
+    def check(step):
+        ri, rs, rx = _sorted(g[f"d{step}_irn"], g[f"d{step}_sc"], g[f"d{step}_x"])
+        oi, os_, ox = _sorted(irn, sc, x)
+        np.testing.assert_array_equal(oi, ri)
+        np.testing.assert_array_equal(os_[:, 0], rs[:, 0])
+        np.testing.assert_array_equal(os_[:, 4], rs[:, 4])
+        np.testing.assert_allclose(os_[:, 1], rs[:, 1], rtol=1e-10)
+        np.testing.assert_allclose(ox, rx, rtol=0, atol=1e-10)
+
+    if 0 in dumps:
+        check(0)
+    est = np.zeros(42)
+    blocks = []
+    for j in range(1, len(g["pop"]) + 1):
+        x, dpsi, sc, irn, nt, est = s.dmc_step(x, dpsi, sc, irn, opcomp=(j % nstep == 0),
+                                                est=est, metropolis=True)
+        assert nt == tuple(int(v) for v in g["pop"][j - 1]), f"step {j}"
+        if j in dumps:
+            check(j)
+        if j % nstep == 0:
+            e = est.reshape(14, 3)
+            blocks.append(e[:13, 0] / e[:13, 2])
+            est = np.zeros(42)
+    np.testing.assert_allclose(np.array(blocks), g["est"][:, :13, 0], rtol=1e-8, atol=1e-300)
+
+
+def test_metropolis_dmc_stack_overflow(oracle_mod):
+    """At dt = 0.002 the reference's Metropolis DMC overflows its stacks (capacity 10 nwalk,
+    stack.f90:88-91) in step 6; the oracle's population trace matches up to there and its
+    sixth step overflows the same capacity."""
+    g = golden("mdmc_N14_overflow.npz")
+    s, x, dpsi, sc, irn = _run_reference_trajectory(oracle_mod, g)
+    cap = int(g["capacity"])
+    for j, ref in enumerate(g["pop"], 1):
+        x, dpsi, sc, irn, nt, _ = s.dmc_step(x, dpsi, sc, irn, cap=cap, metropolis=True)
+        assert nt == tuple(int(v) for v in ref), f"step {j}"
+    with pytest.raises(RuntimeError):
+        s.dmc_step(x, dpsi, sc, irn, cap=cap, metropolis=True)
