@@ -5,7 +5,7 @@
 // largest system first (largest-first order keeps the tail short).  Sizes come from the
 // device population words, so that the whole step can be captured in a graph.
 #include "dmc_batch.h"
-#include "dmc_sm.h"
+#include "dmc_sm2.h"
 
 namespace dmc {
 
@@ -34,16 +34,39 @@ __device__ __forceinline__ int next_item(int *counter, int tid) {
         }                                                                                   \
     }
 
-// moves of chunk c (moves c*k .. min((c+1)*k, npart) - 1 of every walker)
+// moves of chunk c (moves c*k .. min((c+1)*k, npart) - 1 of every walker).  FP32 systems
+// that the two-walker moves take (sm_moves_pair) are dequeued by pairs of walkers, so that
+// every walker runs the code it runs in a per-system launch.
 template <typename R, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_batch_moves(BatchArgs a, int chunk, int *counters,
                                                           int stride) {
     extern __shared__ double smem[];
     const int tid = threadIdx.x & 63;
     double *sm = smem + size_t(threadIdx.x >> 6) * stride;
+    if constexpr (sizeof(R) == 4) {
+        for (int s = 0; s < a.nsys; ++s) {
+            const BatchSys &b = a.sys[s];
+            if (chunk * b.k >= b.npart || !mv_pair_fits(b.nup, b.S.nq)) continue;
+            const int n = b.T.nt[0];
+            int *ctr = counters + s;
+            const int m0 = chunk * b.k, m1 = min(m0 + b.k, b.npart);
+#define DMC_PAIR(NUP)                                                                        \
+    for (int p; 2 * (p = next_item(ctr, tid)) < n;) {                                        \
+        sm_moves_pair<NUP>(b.S, sm, p, n, tid, m0 ? b.B.x : b.A.x, b.B.x,                    \
+                           m0 ? b.B.irn : b.A.irn, b.B.irn, b.A.sinv, m0, m1, b.T);          \
+        wave_sync();                                                                         \
+    }
+            switch (b.nup) {
+                case 19: DMC_PAIR(19) break;
+                case 33: DMC_PAIR(33) break;
+            }
+#undef DMC_PAIR
+        }
+    }
 #define DMC_MOVES(NUP)                                                                      \
     {                                                                                       \
         const int m0 = chunk * b.k, m1 = min(m0 + b.k, b.npart);                            \
+        if (sizeof(R) == 4 && mv_pair_fits(b.nup, b.S.nq)) break;                           \
         sm_moves_walker_any<NUP, R>(b.S, sm, w, tid, m0 ? b.B.x : b.A.x, b.B.x,             \
                                 m0 ? b.B.irn : b.A.irn, b.B.irn, b.A.sinv, m0, m1, b.T);    \
     }
@@ -96,8 +119,8 @@ size_t batch_wave_lds(const BatchHost &h, BatchPhase ph) {
                 switch (b.nup) {
                     case 1: v = mv_lds_bytes<1, float>(nq); break;
                     case 7: v = mv_lds_bytes<7, float>(nq); break;
-                    case 19: v = mv_lds_bytes<19, float>(nq); break;
-                    case 33: v = mv_lds_bytes<33, float>(nq); break;
+                    case 19: v = mv_pair_fits(19, nq) ? mv_pair_lds_bytes<19>() : mv_lds_bytes<19, float>(nq); break;
+                    case 33: v = mv_pair_fits(33, nq) ? mv_pair_lds_bytes<33>() : mv_lds_bytes<33, float>(nq); break;
                 }
             } else {
                 switch (b.nup) {

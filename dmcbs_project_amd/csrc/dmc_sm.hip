@@ -23,7 +23,7 @@
 // lookups, gradients and the Sherman-Morrison updates run in FP32 on an FP32 inverse and
 // FP32 phases; the refresh, the determinant-ratio sum behind the node test, positions,
 // weights and accumulators stay FP64.
-#include "dmc_sm.h"
+#include "dmc_sm2.h"
 
 namespace dmc {
 
@@ -52,6 +52,19 @@ __global__ __launch_bounds__(64, 2) void k_sm_moves64(DevSys S, const double *__
     if (w >= T.nt[0]) return;
     sm_moves_walker<NUP, double, NQC>(S, sm, w, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g,
                                       m0, m1, T);
+}
+
+// FP32 moves of two walkers per wave (workgroup p: walkers 2p and 2p + 1)
+template <int NUP>
+__global__ __launch_bounds__(64) void k_sm_moves32x2(DevSys S, const double *__restrict__ xin,
+                                                     double *xout, const uint64_t *irn_in,
+                                                     uint64_t *irn_out, char *sinv_g, int m0,
+                                                     int m1, DevStep T) {
+    extern __shared__ double sm[];
+    const int nw = T.nt[0];
+    const int wp = blockIdx.x;
+    if (2 * wp >= nw) return;
+    sm_moves_pair<NUP>(S, sm, wp, nw, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g, m0, m1, T);
 }
 
 // FP64 rebuild of the inverse of positions x (every krefresh moves)
@@ -97,6 +110,14 @@ int sm_moves_t(const DevSys &S, const double *xin, double *xout, const uint64_t 
                uint64_t *irn_out, char *sinv, int m0, int m1, const DevStep &T, int grid,
                hipStream_t st) {
     if (S.nq > 64) return hipErrorInvalidValue;  // four projection chunks of 32
+    if constexpr (NUP >= 17) {
+        if (S.sinv_f32 && mv_pair_fits(NUP, S.nq)) {
+            hipLaunchKernelGGL(k_sm_moves32x2<NUP>, dim3((grid + 1) / 2), dim3(64),
+                               mv_pair_lds_bytes<NUP>(), st, S, xin, xout, irn_in, irn_out,
+                               sinv, m0, m1, T);
+            return hipGetLastError();
+        }
+    }
     const bool one = mv_chunks(S.nq) == 1;
     if (S.sinv_f32)
         return one ? sm_moves_tq<NUP, float, 1>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st)

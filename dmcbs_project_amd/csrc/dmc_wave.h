@@ -120,8 +120,12 @@ __device__ __forceinline__ int dpp_bcast(int v, int k) {
                   : __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
 }
 // the same row exchanges with bound_ctrl set (every source lane of these patterns is in the
-// row), so that the compiler folds each exchange into the add (v_add_f32_dpp)
+// row), so that the compiler folds each exchange into the add (v_add_f32_dpp).  The adds are
+// kept apart from the product that may form v (no contraction into an FMA): a fused first
+// step would round the lane's own term and its partner's differently, and the lanes of a row
+// would end with totals that differ in the last bits.
 __device__ __forceinline__ float row_sum16(float v) {
+#pragma clang fp contract(off)
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));

@@ -290,3 +290,39 @@ def test_mixed_ratio_error_against_fp64():
               f"{np.percentile(rerr, 99):.2e} max {rerr.max():.2e}; {err}", flush=True)
         assert np.median(rerr) < 1e-5
         assert err["node"] == 0
+
+
+def test_mixed_drift_tracks_fp64_move_by_move():
+    """The mixed and the FP64 sweeps from the same walkers and random streams (N = 66, k = 66
+    and 33, configurations after three FP64 steps): while a walker's decisions agree, every
+    recorded drift of the mixed sweep agrees with the FP64 one to 1e-5 (absolute, a thousandth
+    of the drift cut).  Among these walkers some have inverse entries in the hundreds, where
+    any inconsistency between the lanes that carry one walker's inverse grows by orders of
+    magnitude within a few moves."""
+    from dmcbs_project_amd import DMC_FP64, DMC_MIXED, DMC_SINGLEPARTICLE, Engine
+    kw = dict(npart=66, norb=5, nwalk=32, dt=0.001, dcut=2.0, popup=1e9, popdn=0.0)
+    eq = Engine(capacity=256, precision=DMC_FP64, movetype=DMC_SINGLEPARTICLE, **kw)
+    eq.init_walkers(7)
+    for _ in range(3):
+        eq.step()
+    w = eq.walkers()
+    for k in (66, 33):
+        rec = {}
+        for prec in (DMC_FP64, DMC_MIXED):
+            e = Engine(capacity=256, precision=prec, movetype=DMC_SINGLEPARTICLE, krefresh=k,
+                       **kw)
+            e.push_walkers(w["x"], w["irn"])
+            e.move_trace(True)
+            e.step1()
+            rec[prec] = e.read_move_trace()
+            e.branch(1.0)
+        r64, r32 = rec[DMC_FP64], rec[DMC_MIXED]
+        n = r64.shape[1]
+        flips = r64[:, :, 2] != r32[:, :, 2]
+        first = np.where(flips.any(axis=1), flips.argmax(axis=1), n)
+        same = np.arange(n)[None, :] <= first[:, None]
+        derr = np.abs(r32[:, :, 7:10] - r64[:, :, 7:10]).max(axis=2)
+        print(f"k={k}: drift err max {derr[same].max():.2e}, walkers with a flip "
+              f"{int((first < n).sum())}", flush=True)
+        assert derr[same].max() < 1e-5
+        assert (first < n).sum() <= 2
