@@ -155,11 +155,35 @@ __device__ __forceinline__ void partner_phases(const DevSys &S, const double *xs
     }
 }
 
-// Pair terms of the mover at pa with a partner at pb (both FP64 positions in registers)
+// Pair terms of the mover at pa with a partner at pb (both FP64 positions in registers), as
+// pair_terms of dmc_sm.h but without a branch: a pair beyond the table range (about half of
+// all pairs of a uniform walker, scattered over the lanes) reads the last interval and takes
+// zero weights, so that the lanes never split and the pair passes stay in one block.
 __device__ __forceinline__ void pair_terms_v(const DevSys &S, const double (&pa)[3],
                                              const double (&pb)[3], float (&dx)[3], float &tb,
                                              float &tj, float &u, float &bv) {
-    pair_terms<float>(S, pa, pb, dx, tb, tj, u, bv);
+    const float r = pair_dist<float>(S, pa, pb, dx);
+    const bool in = r <= float(S.rangeb);
+    const float sc = in ? float(S.scaleb) * r : 0.0f;
+    const int ntab = S.ntab;
+    int idx = int(sc);
+    idx = idx > ntab - 2 ? ntab - 2 : idx;
+    idx = idx < 1 ? 1 : idx;
+    float w0, w1, w2, w3;
+    lagrange_weights<float>(sc - float(idx), w0, w1, w2, w3);
+    const float z = in ? 1.0f : 0.0f;
+    w0 *= z;
+    w1 *= z;
+    w2 *= z;
+    w3 *= z;
+    const float4 a = S.mt32[idx - 1], b = S.mt32[idx], c = S.mt32[idx + 1], d = S.mt32[idx + 2];
+    bv = w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x;
+    const float db = w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y;
+    u = w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z;
+    const float du = w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w;
+    const float rj = r > 1.0e-14f ? r : 1.0e-14f;
+    tb = db * rcp_r<float>(r);
+    tj = du * rcp_r<float>(rj);
 }
 
 // Moves m0..m1-1 of walkers 2 wp and 2 wp + 1 (nw walkers in all; a half whose walker does
@@ -352,26 +376,19 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
 #pragma unroll
         for (int b = 0; b < NUP; ++b) X += wb[b] * plf[pbase + 2 * b];
         // drift at the current position
-        float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-        if (k < NB) {
-            const float f = wgt * ob + oj;
+        float g0, g1, g2;
+        {
+            const float f = k < NB ? wgt * ob + oj : 0.0f;
             g0 = f * odx[0];
             g1 = f * odx[1];
             g2 = f * odx[2];
         }
         if constexpr (BD) {
-            if (own1) {
-                const float f = w31 * e1b + e1j;
-                g0 += f * e1d[0];
-                g1 += f * e1d[1];
-                g2 += f * e1d[2];
-            }
-            if (own2) {
-                const float f = w32 * e2b + e2j;
-                g0 += f * e2d[0];
-                g1 += f * e2d[1];
-                g2 += f * e2d[2];
-            }
+            const float f1 = own1 ? w31 * e1b + e1j : 0.0f;
+            const float f2 = own2 ? w32 * e2b + e2j : 0.0f;
+            g0 += f1 * e1d[0] + f2 * e2d[0];
+            g1 += f1 * e1d[1] + f2 * e2d[1];
+            g2 += f1 * e1d[2] + f2 * e2d[2];
         }
         {
             const float t = cp ? -pc * X : ps * X;
@@ -426,7 +443,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             uns = cp ? so : sn;
             nxc = cp ? cs : co;
             nxs = cp ? sn : so;
-            if (ok) ub[k] = cw * (cp ? uns : unc);
+            ub[k] = cw * (cp ? uns : unc);  // zero past the 2 nq outputs
         }
         wave_sync();
         // pairs at the proposed position (lanes < NA)
@@ -454,11 +471,8 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             jn2 = tj * dx[2];
             dU = uu;
         }
-        if (k < NB) dU -= ou;
-        if constexpr (BD) {
-            if (own1) dU -= e1u;
-            if (own2) dU -= e2u;
-        }
+        dU -= k < NB ? ou : 0.0f;
+        if constexpr (BD) dU -= (own1 ? e1u : 0.0f) + (own2 ? e2u : 0.0f);
         {
             const float t = cp ? -unc * X : uns * X;
             gn0 += cg[0] * t;
@@ -469,7 +483,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
         // n = 33 the proposed position with partner 32 (lane 31)
         const bool lb = BD && k == 31;
         float xdx[3] = {0, 0, 0}, xtb = 0, xtj = 0, xu = 0, xbv = 0;
-        if ((nxt && k < NB) || lb) {
+        if (nxt || BD) {  // wave-uniform; the lanes select their pair
             double pa[3], pb[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -478,20 +492,19 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             }
             float d3[3], b3, j3, u3, bv;
             pair_terms_v(S, pa, pb, d3, b3, j3, u3, bv);
-            if (lb) {
+            const bool keep = nxt && k < NB;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) xdx[d] = d3[d];
-                xtb = b3;
-                xtj = j3;
-                xu = u3;
-                xbv = bv;
-            } else {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) odx[d] = d3[d];
-                ob = b3;
-                oj = j3;
-                ou = u3;
+            for (int d = 0; d < 3; ++d) {
+                xdx[d] = lb ? d3[d] : 0.0f;
+                odx[d] = keep ? d3[d] : odx[d];
             }
+            xtb = lb ? b3 : 0.0f;
+            xtj = lb ? j3 : 0.0f;
+            xu = lb ? u3 : 0.0f;
+            xbv = lb ? bv : 0.0f;
+            ob = keep ? b3 : ob;
+            oj = keep ? j3 : oj;
+            ou = keep ? u3 : ou;
         }
         have_old = nxt;
         float f32 = 0.0f;
@@ -500,17 +513,16 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             float t32 = ok ? ub[k] * plf[pbase + 64] : 0.0f;
             t32 = half_total(t32);
             f32 = alpha0 + half_lane(xbv, 31, upper) + t32;
-            if (lb) {
-                rho_part += double(w32) * double(f32);
-                const float wt = w32 * xtb;
-                gn0 += wt * xdx[0];
-                gn1 += wt * xdx[1];
-                gn2 += wt * xdx[2];
-                jn0 += xtj * xdx[0];
-                jn1 += xtj * xdx[1];
-                jn2 += xtj * xdx[2];
-                dU += xu;
-            }
+            // lane 31 adds partner 32 (its x* terms are zero in the other lanes)
+            rho_part += lb ? double(w32) * double(f32) : 0.0;
+            const float wt = w32 * xtb;
+            gn0 += wt * xdx[0];
+            gn1 += wt * xdx[1];
+            gn2 += wt * xdx[2];
+            jn0 += xtj * xdx[0];
+            jn1 += xtj * xdx[1];
+            jn2 += xtj * xdx[2];
+            dU += xu;
         }
         const double rho = half_total(rho_part);  // FP64 sum: the node test uses its sign
         const float irho = rcp_r<float>(float(rho));
@@ -548,8 +560,8 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             rec[11] = 0.0;
         }
         // Sherman-Morrison step (a rejected move updates by zero)
-        if (k < NA) fb[k] = fnew;
-        if (BD && k == 0) fb[32] = f32;
+        fb[k] = fnew;  // past NUP: never read
+        if constexpr (BD) fb[32] = f32;  // the same value from every lane of the half
         wave_sync();
         {
             float z = 0.0f;
