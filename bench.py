@@ -599,7 +599,7 @@ def main():
     sm = a.workload != "allparticle"
     f = flops_per_walker_step(a.workload, part="moves" if sm else "step")
     achieved = f * main_run["pop_mean"] / (main_run["kernel_ms"] * 1e-3) / 1e12
-    kname = {"mixed": "k_sm_moves32<33,1>", "fp64": "k_sm_moves64<33,1>",
+    kname = {"mixed": "k_sm_moves32x2<33>", "fp64": "k_sm_moves64<33,1>",
              "allparticle": "k_step<33>"}[a.workload]
     peak = PEAK_FP32_TFLOPS if a.workload == "mixed" else PEAK_FP64_TFLOPS
     roof = dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s",
