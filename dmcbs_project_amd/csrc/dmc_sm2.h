@@ -429,57 +429,14 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
                 xn[d] = wrapr(xa[d] + double(dxm[d]), S);
             }
         }
-        // phases: even lanes at the proposed position, odd lanes at the next mover's
         const bool nxt = ip + 1 < m1 && ip + 1 != NUP;
-        float unc, uns, nxc, nxs;
-        {
-            double xq[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) xq[d] = (cp && nxt) ? xs[3 * (ip + 1) + d] : xn[d];
-            float sn, cs;
-            sincos_2pi<float>((cm.x * xq[0] + cm.y * xq[1] + cm.z * xq[2]) * S.eli, sn, cs);
-            const float so = pair_swap(sn), co = pair_swap(cs);
-            unc = cp ? co : cs;
-            uns = cp ? so : sn;
-            nxc = cp ? cs : co;
-            nxs = cp ? sn : so;
-            ub[k] = cw * (cp ? uns : unc);  // zero past the 2 nq outputs
-        }
-        wave_sync();
-        // pairs at the proposed position (lanes < NA)
-        double rho_part = 0.0;
-        float gn0 = 0, gn1 = 0, gn2 = 0, jn0 = 0, jn1 = 0, jn2 = 0, dU = 0, fnew = 0;
-        if (k < NA) {
-            float dx[3], tb, tj, uu, bv;
-            pair_terms_v(S, xn, xp, dx, tb, tj, uu, bv);
-            float f = alpha0 + bv;
-            const float2 *pp = pl + k;
-#pragma unroll 4
-            for (int q = 0; q < nq; ++q) {
-                const float2 v = pp[q * ST];
-                f = fmaf(ub[2 * q], v.x, f);
-                f = fmaf(ub[2 * q + 1], v.y, f);
-            }
-            fnew = f;
-            rho_part = double(wgt) * double(f);
-            const float wt = wgt * tb;
-            gn0 = wt * dx[0];
-            gn1 = wt * dx[1];
-            gn2 = wt * dx[2];
-            jn0 = tj * dx[0];
-            jn1 = tj * dx[1];
-            jn2 = tj * dx[2];
-            dU = uu;
-        }
-        dU -= k < NB ? ou : 0.0f;
-        if constexpr (BD) dU -= (own1 ? e1u : 0.0f) + (own2 ? e2u : 0.0f);
-        {
-            const float t = cp ? -unc * X : uns * X;
-            gn0 += cg[0] * t;
-            gn1 += cg[1] * t;
-            gn2 += cg[2] * t;
-        }
-        // pairs of the next mover of this spin at its current position (lanes < NB), and with
+        // Both pair passes right after the proposal, before the phases and the LDS round trip
+        // of w_q exp(i q.x'): their table loads are then in flight while the phases are formed.
+        // Pass A: the proposed position with partner k (lanes < NA).
+        float adx[3] = {0, 0, 0}, atb = 0, atj = 0, auu = 0, abv = 0;
+        if (k < NA) pair_terms_v(S, xn, xp, adx, atb, atj, auu, abv);
+        const float ou_cur = ou;  // the mover's old u with partner k (pass B overwrites it)
+        // Pass B: the next mover of this spin at its current position (lanes < NB), and with
         // n = 33 the proposed position with partner 32 (lane 31)
         const bool lb = BD && k == 31;
         float xdx[3] = {0, 0, 0}, xtb = 0, xtj = 0, xu = 0, xbv = 0;
@@ -505,6 +462,53 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
             ob = keep ? b3 : ob;
             oj = keep ? j3 : oj;
             ou = keep ? u3 : ou;
+        }
+        // phases: even lanes at the proposed position, odd lanes at the next mover's
+        float unc, uns, nxc, nxs;
+        {
+            double xq[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xq[d] = (cp && nxt) ? xs[3 * (ip + 1) + d] : xn[d];
+            float sn, cs;
+            sincos_2pi<float>((cm.x * xq[0] + cm.y * xq[1] + cm.z * xq[2]) * S.eli, sn, cs);
+            const float so = pair_swap(sn), co = pair_swap(cs);
+            unc = cp ? co : cs;
+            uns = cp ? so : sn;
+            nxc = cp ? cs : co;
+            nxs = cp ? sn : so;
+            ub[k] = cw * (cp ? uns : unc);  // zero past the 2 nq outputs
+        }
+        wave_sync();
+        // pass A's new pair values f_k and gradient terms
+        double rho_part = 0.0;
+        float gn0 = 0, gn1 = 0, gn2 = 0, jn0 = 0, jn1 = 0, jn2 = 0, dU = 0, fnew = 0;
+        if (k < NA) {
+            float f = alpha0 + abv;
+            const float2 *pp = pl + k;
+#pragma unroll 4
+            for (int q = 0; q < nq; ++q) {
+                const float2 v = pp[q * ST];
+                f = fmaf(ub[2 * q], v.x, f);
+                f = fmaf(ub[2 * q + 1], v.y, f);
+            }
+            fnew = f;
+            rho_part = double(wgt) * double(f);
+            const float wt = wgt * atb;
+            gn0 = wt * adx[0];
+            gn1 = wt * adx[1];
+            gn2 = wt * adx[2];
+            jn0 = atj * adx[0];
+            jn1 = atj * adx[1];
+            jn2 = atj * adx[2];
+            dU = auu;
+        }
+        dU -= k < NB ? ou_cur : 0.0f;
+        if constexpr (BD) dU -= (own1 ? e1u : 0.0f) + (own2 ? e2u : 0.0f);
+        {
+            const float t = cp ? -unc * X : uns * X;
+            gn0 += cg[0] * t;
+            gn1 += cg[1] * t;
+            gn2 += cg[2] * t;
         }
         have_old = nxt;
         float f32 = 0.0f;
