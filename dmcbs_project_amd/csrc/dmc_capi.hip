@@ -7,6 +7,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -195,6 +196,13 @@ int upload_tables(dmc_ctx *c) {
     HIPCHK(hipMemcpyAsync(const_cast<float4 *>(c->ds.mt32), mt32.data(), sizeof(float4) * n1,
                           hipMemcpyHostToDevice, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    // The end of a single-particle sweep reads these tables from global memory, one walker per
+    // workgroup (1.52 ms at N = 66 with 32k walkers).  DMC_HPSI_TABLES=lds selects the kernel
+    // that stages them in the LDS of workgroups of eight walkers (1.76 ms), where such a
+    // workgroup fits the device.
+    c->ds.tab_lds = 0;
+    const char *tv = getenv("DMC_HPSI_TABLES");
+    if (c->ds.keep_sinv && tv && !strcmp(tv, "lds")) c->ds.tab_lds = sm_end_tab_fits(s.n, c->ds);
     return 0;
 }
 

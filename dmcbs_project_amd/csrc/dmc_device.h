@@ -20,6 +20,9 @@ struct DevSys {
     // Jastrow's healing distance read 8 bytes per node instead of the 32-byte record
     const double *vt1;
     int jfar;
+    // the end-of-sweep kernel stages the radial tables in the LDS of workgroups of several
+    // walkers (1), or reads them from global memory with one walker per workgroup (0)
+    int tab_lds;
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
     // the same beta values in compact arrays, so that a pair loop touches only the fields it
     // reads (a smaller working set in the vector L1): b, [ntab+1]; {db, lap b}, [ntab+1]

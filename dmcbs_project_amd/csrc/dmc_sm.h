@@ -782,23 +782,31 @@ __device__ __forceinline__ void sm_refresh_walker(const DevSys &S, double *sm, i
 
 // end of the sweep: FP64 hpsi of the final positions B.x, the fresh inverse into B.sinv,
 // weight against the energy of A, estimators and branch draw (state after the moves in
-// B.irn)
-template <int NUP>
-__device__ __forceinline__ void sm_end_walker(const DevSys &S, double *sm, int w, int tid,
-                                              const DevState &A, const DevState &B,
+// B.irn).  TB: the view of the radial tables (TabL: staged in the workgroup's LDS, with the
+// walker's area in the compact layout).
+template <int NUP, class TB>
+__device__ __forceinline__ void sm_end_walker(const DevSys &S, const TB &tb, double *sm, int w,
+                                              int tid, const DevState &A, const DevState &B,
                                               const DevStep &T, int opcomp) {
-    const Layout<NUP> L(S.nq, true);
+    const Layout<NUP> L(S.nq, true, TB::kCompact);
     constexpr int M = 6 * NUP;
     load_q<NUP>(S, sm, L, tid);
     for (int t = tid; t < M; t += 64) sm[L.xs + t] = B.x[size_t(w) * M + t];
     wave_sync();
     const Tile<NUP> Tl(tid);
     PairTile<NUP> rad;
-    const JSums J = jastrow_terms<NUP>(S, sm, L, Tl, tid, &rad);
-    const DetInfo D = build_inverse<NUP>(S, sm, L, Tl, tid, &rad);
+    const JSums J = jastrow_terms<NUP>(S, tb, sm, L, Tl, tid, &rad);
+    const DetInfo D = build_inverse<NUP>(S, tb, sm, L, Tl, tid, &rad);
     store_sinv<NUP>(S, sm, L, D.singular, B.sinv + size_t(w) * sinv_stride(S), tid);
-    grad_lap<NUP>(S, sm, L, Tl, D, J, tid);
+    grad_lap<NUP>(S, tb, sm, L, Tl, D, J, tid);
     finish_walker<NUP>(S, sm, L, A, B, T, w, tid, B.irn[w], opcomp);
+}
+
+template <int NUP>
+__device__ __forceinline__ void sm_end_walker(const DevSys &S, double *sm, int w, int tid,
+                                              const DevState &A, const DevState &B,
+                                              const DevStep &T, int opcomp) {
+    sm_end_walker<NUP>(S, TabG(S), sm, w, tid, A, B, T, opcomp);
 }
 
 }  // namespace dmc

@@ -102,6 +102,107 @@ __device__ __forceinline__ double beta_val(const DevSys &S, double r) {
     return w0 * S.bt1[idx - 1] + w1 * S.bt1[idx] + w2 * S.bt1[idx + 1] + w3 * S.bt1[idx + 2];
 }
 
+// ------------------------------------------------------------------------------------
+// Views of the radial tables that the hpsi pair passes read: beta b [ntab+1], {beta',
+// lap beta} [ntab+1], the potential [ntab+1] and the Jastrow/potential records {v, u, u',
+// u''} of the intervals before jfar.
+//
+// TabG reads the arrays of the system in global memory (through the vector L1), one walker
+// per workgroup.  TabL reads a copy that the workgroup stages in LDS once for all of its
+// walkers (one per wave), with the walkers' own areas in the compact layout beside it:
+// a 64-lane gather of random table entries is one LDS access per lane instead of one L1
+// cache-line access per lane, at LDS latency.
+struct TabG {
+    static constexpr bool kCompact = false;
+    const double *b1, *v1;
+    const double2 *b2;
+    const double4 *j4;
+    __device__ explicit TabG(const DevSys &S) : b1(S.bt1), v1(S.vt1), b2(S.bt2), j4(S.jt) {}
+    __device__ double bt1(int i) const { return b1[i]; }
+    __device__ double vt1(int i) const { return v1[i]; }
+    __device__ double2 bt2(int i) const { return b2[i]; }
+    __device__ double4 jt(int i) const { return j4[i]; }
+};
+
+typedef __attribute__((address_space(3))) const double lds_cdouble;
+
+// offsets (in doubles) of the staged tables at the start of a workgroup's LDS: bt1, vt1,
+// bt2 (pairs), then the records 0 .. jfar + 1 that a stencil before jfar reads (all of them
+// when the Jastrow reaches the end of the table); the walkers' areas start at `total`,
+// a multiple of 16 doubles
+struct TabLayout {
+    int n1, nrec, b1, v1, b2, j4, total;
+    __host__ __device__ explicit TabLayout(int ntab, int jfar) {
+        n1 = ntab + 1;
+        nrec = jfar + 2 < n1 ? jfar + 2 : n1;
+        b1 = 0;
+        v1 = b1 + n1;
+        b2 = (v1 + n1 + 1) / 2 * 2;  // 16-byte aligned pairs
+        j4 = b2 + 2 * n1;
+        total = (j4 + 4 * nrec + 15) / 16 * 16;
+    }
+};
+
+struct TabL {
+    static constexpr bool kCompact = true;
+    lds_cdouble *b1, *v1, *b2, *j4;
+    __device__ TabL(const DevSys &S, const double *base) {
+        const TabLayout TL(S.ntab, S.jfar);
+        lds_cdouble *p = (lds_cdouble *)(base);
+        b1 = p + TL.b1;
+        v1 = p + TL.v1;
+        b2 = p + TL.b2;
+        j4 = p + TL.j4;
+    }
+    __device__ double bt1(int i) const { return b1[i]; }
+    __device__ double vt1(int i) const { return v1[i]; }
+    __device__ double2 bt2(int i) const { return make_double2(b2[2 * i], b2[2 * i + 1]); }
+    __device__ double4 jt(int i) const {
+        return make_double4(j4[4 * i], j4[4 * i + 1], j4[4 * i + 2], j4[4 * i + 3]);
+    }
+};
+
+// copy of the tables into the workgroup's LDS at base, by all of its threads; the caller
+// synchronises the workgroup before the first read
+__device__ __forceinline__ void stage_tables(const DevSys &S, double *base, int tid, int nthr) {
+    const TabLayout TL(S.ntab, S.jfar);
+    for (int i = tid; i < TL.n1; i += nthr) {
+        base[TL.b1 + i] = S.bt1[i];
+        base[TL.v1 + i] = S.vt1[i];
+        const double2 v = S.bt2[i];
+        base[TL.b2 + 2 * i] = v.x;
+        base[TL.b2 + 2 * i + 1] = v.y;
+    }
+    for (int i = tid; i < TL.nrec; i += nthr) {
+        const double4 v = S.jt[i];
+        base[TL.j4 + 4 * i] = v.x;
+        base[TL.j4 + 4 * i + 1] = v.y;
+        base[TL.j4 + 4 * i + 2] = v.z;
+        base[TL.j4 + 4 * i + 3] = v.w;
+    }
+}
+
+// walkers (one per wave) per workgroup of the kernels that stage the tables in LDS
+constexpr int kTabWaves = 8;
+
+template <class TB>
+__device__ __forceinline__ double4 lookup4j(const TB &tb, int idx, double w0, double w1,
+                                            double w2, double w3) {
+    const double4 a = tb.jt(idx - 1), b = tb.jt(idx), c = tb.jt(idx + 1), d = tb.jt(idx + 2);
+    return make_double4(w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x,
+                        w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y,
+                        w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z,
+                        w0 * a.w + w1 * b.w + w2 * c.w + w3 * d.w);
+}
+
+template <class TB>
+__device__ __forceinline__ double beta_val(const TB &tb, const DevSys &S, double r) {
+    if (S.nobeta || r > S.rangeb) return 0.0;
+    double w0, w1, w2, w3;
+    const int idx = lagrange(S.scaleb * r, S.ntab, w0, w1, w2, w3);
+    return w0 * tb.bt1(idx - 1) + w1 * tb.bt1(idx) + w2 * tb.bt1(idx + 1) + w3 * tb.bt1(idx + 2);
+}
+
 // Sum over the 64 lanes of a wave, returned as a wave-uniform value.  Within each 16-lane
 // row by DPP lane exchanges (xor 1, xor 2, mirror within 8, mirror within 16), then rows
 // combined by row_bcast:15 (rows 1, 3 add the last lane of the row before) and row_bcast:31
@@ -248,8 +349,46 @@ __device__ __forceinline__ void phases_all(const DevSys &S, const double *xs, co
     }
 }
 
+// exp(i q.r) of every particle for the q vectors q0 <= q < q1 into ph[(q - q0) * NP + p]:
+// the same arithmetic per element as phases_all (one chunk of the q vectors at a time, for
+// the compact layout)
+template <int NUP>
+__device__ __forceinline__ void phases_range(const DevSys &S, const double *xs, const int4 *qm,
+                                             double2 *ph, int q0, int q1, int tid) {
+    constexpr int NP = 2 * NUP;
+    if (tid < NP) {
+        double pc[3][5], ps[3][5];
+        axis_powers(S, xs + 3 * tid, pc, ps);
+        for (int q = q0; q < q1; ++q) {
+            const int4 m = qm[q];
+            double cx, sx, cy, sy, cz, sz;
+            cpow(pc[0], ps[0], m.x, cx, sx);
+            cpow(pc[1], ps[1], m.y, cy, sy);
+            cpow(pc[2], ps[2], m.z, cz, sz);
+            const double cxy = cx * cy - sx * sy, sxy = sx * cy + cx * sy;
+            ph[(q - q0) * NP + tid] = make_double2(cxy * cz - sxy * sz, sxy * cz + cxy * sz);
+        }
+    }
+    if constexpr (NP > 64) {
+        const int nqc = q1 - q0;
+        for (int t = tid; t < (NP - 64) * nqc; t += 64) {
+            const int e = t / nqc, qq = t - e * nqc, p = 64 + e;
+            const int4 m = qm[q0 + qq];
+            const double *pos = xs + 3 * p;
+            const double th = S.twopi_l * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            ph[qq * NP + p] = make_double2(cs, sn);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // LDS layout of one walker (offsets in doubles)
+//
+// Compact layout (hpsi kernels that share LDS-staged tables between the walkers of a
+// workgroup): the lean layout with the phases of qch q vectors at a time and the partial
+// sums of one spin at a time, so that eight walkers and the tables fit in the LDS of a CU.
 //
 // Full layout (single-particle ratio, parameter adjoint): the phases of every particle and
 // the inverse stay in LDS after build_inverse.  Lean layout (hpsi, step and refresh
@@ -263,12 +402,18 @@ struct Layout {
     // row stride of the inverse and of the partial sums: = 8 (mod 16), so that the 8 x 8
     // lane grid reading a tile (rows c + 8b, columns r + 8a) hits distinct banks
     static constexpr int SP = (NUP + 7) / 16 * 16 + 8;
+    // q vectors per chunk of phases in the compact layout
+    static constexpr int kQChunk = 8;
     int xs, qc, qm, dp, xn, prow, mcol, ints, work, sinv, est, scal, total;
+    int qch;  // q vectors whose phases the work area holds at once
     bool lean;
-    __host__ __device__ explicit Layout(int nq, bool lean_ = false) : lean(lean_) {
-        const int phs = 2 * NP * nq;          // phases, [nq][NP] complex
+    __host__ __device__ explicit Layout(int nq, bool lean_ = false, bool compact = false)
+        : lean(lean_ || compact) {
+        qch = compact && nq > kQChunk ? kQChunk : nq;
+        const int phs = 2 * NP * qch;         // phases, [qch][NP] complex
         const int inv = NUP * SP;             // Sinv(j, i) at [j * SP + i]
-        const int part = 6 * 8 * SP;          // partial sums of one spin, [6][8][SP]
+        // partial sums, [spin][3][8][SP]: both spins, or one at a time (compact)
+        const int part = (compact ? 3 : 6) * 8 * SP;
         xs = 0;
         qc = xs + 3 * NP;                     // per q: w, -w q (3), -w q^2
         qm = qc + 5 * nq;                     // integer q vectors (int4)
@@ -303,6 +448,20 @@ struct Layout {
 
 template <int NUP>
 size_t lds_bytes(int nq, bool lean = false) { return sizeof(double) * Layout<NUP>(nq, lean).total; }
+
+// per-walker stride (doubles) of the compact layout: whole 128-byte lines
+template <int NUP>
+__host__ __device__ inline int compact_stride(int nq) {
+    return (Layout<NUP>(nq, true, true).total + 15) / 16 * 16;
+}
+
+// LDS of a workgroup of the table-staging kernels: the tables, then one walker area in the
+// compact layout per wave
+template <int NUP>
+size_t tab_lds_bytes(const DevSys &S) {
+    return sizeof(double) * (size_t(TabLayout(S.ntab, S.jfar).total) +
+                             size_t(kTabWaves) * compact_stride<NUP>(S.nq));
+}
 
 // Load q coefficients into LDS (all lanes)
 template <int NUP>
@@ -390,6 +549,25 @@ __device__ __forceinline__ void pivot_argmax(double &best, int &brow, int tid) {
     }
 }
 
+// per-lane partial sums of the particles of one spin into the work area, [3][8][SP]: the up
+// particles ri[a] of the lane's tile rows by lane column (spin 0), or the down particles
+// cj[a] of its tile columns by lane row (spin 1); a lane over particles then sums the 8
+// entries of its particle
+template <int NUP>
+__device__ __forceinline__ void put_spin_part(double *part, const Tile<NUP> &T, int spin,
+                                              const double (*p)[3]) {
+    constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP;
+#pragma unroll
+    for (int a = 0; a < TR; ++a) {
+        const bool v = spin ? T.cv[a] : T.rv[a];
+        const int k = spin ? T.cj[a] : T.ri[a], l = spin ? T.r : T.c;
+        if (v) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) part[(d * 8 + l) * SP + k] = p[a][d];
+        }
+    }
+}
+
 // the radial part alpha_1 + beta(r) of a lane's tile of the pairing matrix
 template <int NUP>
 struct PairTile {
@@ -405,8 +583,10 @@ struct DetInfo {
 // Phases A-D: plane-wave phases of the positions in LDS (xs), the pairing matrix, its
 // inverse, log|det| and sign.  Leaves the phases (work area) and the inverse (sinv,
 // Sinv(j, i) at [j * SP + i]) in LDS.
-template <int NUP>
-__device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
+// TB: the view of the beta table (when rad is not given); with a compact-layout view the
+// phases are built and contracted qch q vectors at a time.
+template <int NUP, class TB>
+__device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, double *sm,
                                                  const Layout<NUP> &L, const Tile<NUP> &T,
                                                  int tid, const PairTile<NUP> *rad = nullptr) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
@@ -423,8 +603,10 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
 
     // Phase A: plane-wave phases of every particle
     // q-major: ph[q * NP + p], so that lanes over particles read consecutive words
-    phases_all<NUP>(S, xs, qm, ph, tid);
-    wave_sync();
+    if constexpr (!TB::kCompact) {
+        phases_all<NUP>(S, xs, qm, ph, tid);
+        wave_sync();
+    }
 
     // Phase B: pairing-matrix tile S(i, j) = alpha_1 + beta(|r_ij|) + sum_q w_q cos(q.r_ij);
     // the radial part alpha_1 + beta comes from the Jastrow pass when it ran first (rad)
@@ -444,27 +626,42 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
                 const double dx = wrapd(xi[d] - xj[d], S);
                 r2 += dx * dx;
             }
-            Sm[a][b] = S.alpha0 + beta_val(S, sqrt(r2));
+            Sm[a][b] = S.alpha0 + beta_val(tb, S, sqrt(r2));
         }
-    for (int q = 0; q < nq; ++q) {
-        const double w = qc[5 * q];
-        double2 zi[TR], zj[TR];
+    // the plane-wave terms of q vectors q0 <= q < q1, whose phases are at ph[(q - q0) * NP]
+    auto add_waves = [&](int q0, int q1) __attribute__((always_inline)) {
+        for (int q = q0; q < q1; ++q) {
+            const double w = qc[5 * q];
+            const double2 *pq = ph + (q - q0) * NP;
+            double2 zi[TR], zj[TR];
 #pragma unroll
-        for (int a = 0; a < TR; ++a) {
-            zi[a] = ph[q * NP + ri[a]];
-            zi[a].x *= w;
-            zi[a].y *= w;
-            zj[a] = ph[q * NP + NUP + cj[a]];
-        }
-#pragma unroll
-        for (int a = 0; a < TR; ++a)
-#pragma unroll
-            for (int b = 0; b < TR; ++b) {
-                Sm[a][b] = fma(zi[a].x, zj[b].x, Sm[a][b]);
-                Sm[a][b] = fma(zi[a].y, zj[b].y, Sm[a][b]);
+            for (int a = 0; a < TR; ++a) {
+                zi[a] = pq[ri[a]];
+                zi[a].x *= w;
+                zi[a].y *= w;
+                zj[a] = pq[NUP + cj[a]];
             }
+#pragma unroll
+            for (int a = 0; a < TR; ++a)
+#pragma unroll
+                for (int b = 0; b < TR; ++b) {
+                    Sm[a][b] = fma(zi[a].x, zj[b].x, Sm[a][b]);
+                    Sm[a][b] = fma(zi[a].y, zj[b].y, Sm[a][b]);
+                }
+        }
+    };
+    if constexpr (TB::kCompact) {
+        for (int q0 = 0; q0 < nq; q0 += L.qch) {
+            const int q1 = q0 + L.qch < nq ? q0 + L.qch : nq;
+            phases_range<NUP>(S, xs, qm, ph, q0, q1, tid);
+            wave_sync();
+            add_waves(q0, q1);
+            wave_sync();  // the next chunk's phases, or the elimination scratch, go there
+        }
+    } else {
+        add_waves(0, nq);
+        wave_sync();  // the lean layout puts the elimination scratch over the phases
     }
-    wave_sync();  // the lean layout puts the elimination scratch over the phases
 
     // Phase C: Gauss-Jordan inversion with partial pivoting, rows never moved
     double *prow = sm + L.prow, *mcol = sm + L.mcol;
@@ -592,6 +789,13 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
     return DetInfo{logdet, sgn, singular};
 }
 
+template <int NUP>
+__device__ __forceinline__ DetInfo build_inverse(const DevSys &S, double *sm,
+                                                 const Layout<NUP> &L, const Tile<NUP> &T,
+                                                 int tid, const PairTile<NUP> *rad = nullptr) {
+    return build_inverse<NUP>(S, TabG(S), sm, L, T, tid, rad);
+}
+
 // Jastrow and potential terms of the pair loop of getphi (coshpot.f90:165-202), opposite
 // spins: per particle, lane p < NUP holds the Jastrow gradient of up particle p (j[0]) and
 // down particle p (j[1]); the sums U, V and d2U are wave-uniform.  Runs before the pairing
@@ -603,8 +807,8 @@ struct JSums {
     double U, V, D2U;
 };
 
-template <int NUP>
-__device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
+template <int NUP, class TB>
+__device__ __forceinline__ JSums jastrow_terms(const DevSys &S, const TB &tb, double *sm,
                                                const Layout<NUP> &L, const Tile<NUP> &T,
                                                int tid, PairTile<NUP> *rad = nullptr) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP;
@@ -636,17 +840,17 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
             // r and 1/r from one reciprocal square root, r floored at 1e-14 (coshpot.f90:167;
             // the floor also keeps coincident particles finite in the beta lookup)
             const double irj = rsqrt_nr(fmax(r2, 1.0e-28)), rj = r2 * irj, rr = rj;
-            if (rad) rad->v[a][b] = S.alpha0 + beta_val(S, rr);
+            if (rad) rad->v[a][b] = S.alpha0 + beta_val(tb, S, rr);
             if (rj < S.range) {
                 double w0, w1, w2, w3;
                 const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
                 if (idx >= S.jfar) {
                     // past the healing distance u, u' and u'' are zero on the whole stencil:
                     // the potential alone, the same arithmetic as the record's first field
-                    V += w0 * S.vt1[idx - 1] + w1 * S.vt1[idx] + w2 * S.vt1[idx + 1] +
-                         w3 * S.vt1[idx + 2];
+                    V += w0 * tb.vt1(idx - 1) + w1 * tb.vt1(idx) + w2 * tb.vt1(idx + 1) +
+                         w3 * tb.vt1(idx + 2);
                 } else {
-                    const double4 jv = lookup4(S.jt, idx, w0, w1, w2, w3);
+                    const double4 jv = lookup4j(tb, idx, w0, w1, w2, w3);
                     V += jv.x;
                     U += jv.y;
                     const double t = jv.z * irj * S.invsqrtm;
@@ -662,35 +866,60 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
             // (unfenced, the straight-line loop issues every pair's loads at once)
             __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-    for (int a = 0; a < TR; ++a) {
-        if (T.rv[a]) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + T.ri[a]] = pu[a][d];
-        }
-        if (T.cv[a]) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + T.cj[a]] = pdn[a][d];
-        }
-    }
-    wave_sync();
     JSums J;
     const int pl = tid < NUP ? tid : 0;
+    if constexpr (TB::kCompact) {
+        // one spin at a time through the work area: the same sums in the same order
 #pragma unroll
-    for (int spin = 0; spin < 2; ++spin)
+        for (int spin = 0; spin < 2; ++spin) {
+            put_spin_part<NUP>(part, T, spin, spin ? pdn : pu);
+            wave_sync();
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double v = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double v = 0.0;
 #pragma unroll
-            for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
-            pin(v);
-            J.j[spin][d] = v;
+                for (int l = 0; l < 8; ++l) v += part[(d * 8 + l) * SP + pl];
+                pin(v);
+                J.j[spin][d] = v;
+            }
+            wave_sync();
         }
+    } else {
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+            if (T.rv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + T.ri[a]] = pu[a][d];
+            }
+            if (T.cv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + T.cj[a]] = pdn[a][d];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int spin = 0; spin < 2; ++spin)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
+                pin(v);
+                J.j[spin][d] = v;
+            }
+    }
     J.U = wave_sum(U);
     J.V = wave_sum(V);
     J.D2U = wave_sum(D2U);
     wave_sync();
     return J;
+}
+
+template <int NUP>
+__device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
+                                               const Layout<NUP> &L, const Tile<NUP> &T,
+                                               int tid, PairTile<NUP> *rad = nullptr) {
+    return jastrow_terms<NUP>(S, TabG(S), sm, L, T, tid, rad);
 }
 
 // Phases E-G: gradients and laplacian of log det through the inverse, Jastrow and
@@ -701,8 +930,9 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, double *sm,
 // hold one set of partial sums at a time.  Partial sums travel as [quantity][lane row or
 // column][particle] (row stride SP): the stores of the lane grid and the loads of lanes over
 // particles both hit distinct banks.
-template <int NUP>
-__device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, const Tile<NUP> &T,
+template <int NUP, class TB>
+__device__ void grad_lap(const DevSys &S, const TB &tb, double *sm, const Layout<NUP> &L,
+                         const Tile<NUP> &T,
                          const DetInfo &D, const JSums &J, int tid) {
     constexpr int TR = Layout<NUP>::TR, SP = Layout<NUP>::SP, NP = Layout<NUP>::NP;
     constexpr int PS = 3 * 8 * SP;  // partial sums of one spin, [3][8][SP]
@@ -753,8 +983,8 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
                 if (!S.nobeta && rr <= S.rangeb) {
                     double w0, w1, w2, w3;
                     const int idx = lagrange(S.scaleb * rr, S.ntab, w0, w1, w2, w3);
-                    const double2 A = S.bt2[idx - 1], B = S.bt2[idx], C = S.bt2[idx + 1],
-                                  D = S.bt2[idx + 2];
+                    const double2 A = tb.bt2(idx - 1), B = tb.bt2(idx), C = tb.bt2(idx + 1),
+                                  D = tb.bt2(idx + 2);
                     const double2 bv = make_double2(w0 * A.x + w1 * B.x + w2 * C.x + w3 * D.x,
                                                     w0 * A.y + w1 * B.y + w2 * C.y + w3 * D.y);
                     const double t = sv[a][b] * bv.x * ir;
@@ -767,32 +997,53 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
                 }
                 __builtin_amdgcn_sched_barrier(0);  // one pair's table loads in flight
             }
+        if constexpr (TB::kCompact) {
 #pragma unroll
-        for (int a = 0; a < TR; ++a) {
-            if (rv[a]) {
+            for (int spin = 0; spin < 2; ++spin) {
+                put_spin_part<NUP>(part, T, spin, spin ? pdn : pu);
+                wave_sync();
 #pragma unroll
-                for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = pu[a][d];
+                for (int d = 0; d < 3; ++d) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) v += part[(d * 8 + l) * SP + pl];
+                    gs[spin][d] = v;
+                }
+                wave_sync();
             }
-            if (cv[a]) {
+        } else {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = pdn[a][d];
+            for (int a = 0; a < TR; ++a) {
+                if (rv[a]) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = pu[a][d];
+                }
+                if (cv[a]) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = pdn[a][d];
+                }
             }
         }
     }
-    wave_sync();
+    if constexpr (!TB::kCompact) {
+        wave_sync();
 #pragma unroll
-    for (int spin = 0; spin < 2; ++spin)
+        for (int spin = 0; spin < 2; ++spin)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double v = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double v = 0.0;
 #pragma unroll
-            for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
-            gs[spin][d] = v;
-        }
+                for (int l = 0; l < 8; ++l) v += part[spin * PS + (d * 8 + l) * SP + pl];
+                gs[spin][d] = v;
+            }
+    }
     wave_sync();
-    phases_all<NUP>(S, xs, reinterpret_cast<const int4 *>(sm + L.qm),
-                    reinterpret_cast<double2 *>(sm + L.work), tid);
-    wave_sync();
+    const int4 *qmv = reinterpret_cast<const int4 *>(sm + L.qm);
+    double2 *phw = reinterpret_cast<double2 *>(sm + L.work);
+    if constexpr (!TB::kCompact) {
+        phases_all<NUP>(S, xs, qmv, phw, tid);
+        wave_sync();
+    }
 
     double gu[TR][3], gd[TR][3];
 #pragma unroll
@@ -801,14 +1052,23 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
         for (int d = 0; d < 3; ++d) gu[a][d] = gd[a][d] = 0.0;
     // plane-wave part: Sum_j Sinv(j, i) sin(q.r_ij) = s_i X_i - c_i Y_i with X = C^T Sinv
     // for the up particles, and the same with Z, W for the down particles, one pass over q
+    // (the compact layout: the phases of chunks of qch q vectors, built in turn)
+    const int qstep = TB::kCompact ? L.qch : nq;
+    for (int q0 = 0; q0 < nq; q0 += qstep) {
+    const int q1 = q0 + qstep < nq ? q0 + qstep : nq;
+    if constexpr (TB::kCompact) {
+        if (q0 > 0) wave_sync();  // done with the previous chunk
+        phases_range<NUP>(S, xs, qmv, phw, q0, q1, tid);
+        wave_sync();
+    }
 #pragma unroll 1
-    for (int q = 0; q < nq; ++q) {
+    for (int q = q0; q < q1; ++q) {
         const double *cq = qc + 5 * q;
         double2 zi[TR], zj[TR];
 #pragma unroll
         for (int a = 0; a < TR; ++a) {
-            zi[a] = ph[q * NP + ri[a]];
-            zj[a] = ph[q * NP + NUP + cj[a]];
+            zi[a] = ph[(q - q0) * NP + ri[a]];
+            zj[a] = ph[(q - q0) * NP + NUP + cj[a]];
         }
         double ls = 0.0;
 #pragma unroll
@@ -841,32 +1101,50 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
             gd[b][2] -= cq[3] * u;
         }
     }
+    }
     wave_sync();  // done with the phases
 
     // determinant gradients per particle
+    if constexpr (TB::kCompact) {
 #pragma unroll
-    for (int a = 0; a < TR; ++a) {
-        if (rv[a]) {
+        for (int spin = 0; spin < 2; ++spin) {
+            put_spin_part<NUP>(part, T, spin, spin ? gd : gu);
+            wave_sync();
 #pragma unroll
-            for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = gu[a][d];
+            for (int d = 0; d < 3; ++d) {
+                double g = 0.0;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) g += part[(d * 8 + l) * SP + pl];
+                gs[spin][d] += g;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            wave_sync();
         }
-        if (cv[a]) {
+    } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = gd[a][d];
+        for (int a = 0; a < TR; ++a) {
+            if (rv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[(d * 8 + c) * SP + ri[a]] = gu[a][d];
+            }
+            if (cv[a]) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) part[PS + (d * 8 + r) * SP + cj[a]] = gd[a][d];
+            }
         }
+        wave_sync();
+#pragma unroll
+        for (int spin = 0; spin < 2; ++spin)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double g = 0.0;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) g += part[spin * PS + (d * 8 + l) * SP + pl];
+                gs[spin][d] += g;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        wave_sync();  // dp shares the work area
     }
-    wave_sync();
-#pragma unroll
-    for (int spin = 0; spin < 2; ++spin)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double g = 0.0;
-#pragma unroll
-            for (int l = 0; l < 8; ++l) g += part[spin * PS + (d * 8 + l) * SP + pl];
-            gs[spin][d] += g;
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    wave_sync();  // dp shares the work area
     double du2 = 0.0, dudp = 0.0;
     if (tid < NUP) {
 #pragma unroll
@@ -893,6 +1171,13 @@ __device__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L, cons
         sm[L.scal + 4] = logdet;
     }
     wave_sync();
+}
+
+template <int NUP>
+__device__ __forceinline__ void grad_lap(const DevSys &S, double *sm, const Layout<NUP> &L,
+                                         const Tile<NUP> &T, const DetInfo &D, const JSums &J,
+                                         int tid) {
+    grad_lap<NUP>(S, TabG(S), sm, L, T, D, J, tid);
 }
 
 // copy the inverse left in LDS by build_inverse to global memory, dense [j][i], as FP64
@@ -930,14 +1215,20 @@ __host__ __device__ inline size_t sinv_stride(const DevSys &S) {
     return (sinv_bytes(S) + 127) / 128 * 128;
 }
 
+template <int NUP, class TB = TabG>
+__device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
+                                           int tid, const TB &tb) {
+    const Tile<NUP> T(tid);
+    PairTile<NUP> rad;
+    const JSums J = jastrow_terms<NUP>(S, tb, sm, L, T, tid, &rad);
+    const DetInfo D = build_inverse<NUP>(S, tb, sm, L, T, tid, &rad);
+    grad_lap<NUP>(S, tb, sm, L, T, D, J, tid);
+}
+
 template <int NUP>
 __device__ __forceinline__ void hpsi_block(const DevSys &S, double *sm, const Layout<NUP> &L,
                                            int tid) {
-    const Tile<NUP> T(tid);
-    PairTile<NUP> rad;
-    const JSums J = jastrow_terms<NUP>(S, sm, L, T, tid, &rad);
-    const DetInfo D = build_inverse<NUP>(S, sm, L, T, tid, &rad);
-    grad_lap<NUP>(S, sm, L, T, D, J, tid);
+    hpsi_block<NUP>(S, sm, L, tid, TabG(S));
 }
 
 // Jastrow ratio exp(sum_j u(|x' - r_j|) - u(|r_i - r_j|)) over the opposite-spin partners

@@ -328,3 +328,44 @@ def test_mixed_drift_tracks_fp64_move_by_move():
               f"{int((first < n).sum())}", flush=True)
         assert derr[same].max() < 1e-5
         assert (first < n).sum() <= 2
+
+
+@pytest.mark.parametrize("n,norb,nw,k,prec", [(14, 3, 100, 14, "fp64"), (38, 4, 40, 19, "mixed"),
+                                              (66, 5, 52, 66, "mixed"), (66, 5, 20, 33, "fp64")])
+def test_lds_table_end_kernel(monkeypatch, n, norb, nw, k, prec):
+    """the end-of-sweep kernel that stages the radial tables in LDS for workgroups of eight
+    walkers (DMC_HPSI_TABLES=lds, compact walker layout: phases of eight q vectors at a time,
+    partial sums one spin at a time) against the default one-walker kernel: the same sums in
+    the same order, so populations, positions and local energies are bit-identical; with
+    populations that leave the last workgroup partly empty, and the FP64 sweep against the
+    oracle as well"""
+    from dmcbs_project_amd import DMC_FP64, DMC_MIXED
+    precision = DMC_FP64 if prec == "fp64" else DMC_MIXED
+    runs = []
+    for tables in ("global", "lds"):
+        monkeypatch.setenv("DMC_HPSI_TABLES", tables)
+        e, s = engines(n, norb, nw, k, precision, dcut=2.0)
+        x, irn = s.init_walkers(5, nw)
+        e.push_walkers(x, irn)
+        for _ in range(3):
+            e.step()
+        runs.append(e.walkers())
+        e.close()
+    a, b = runs
+    for key in ("irn", "x", "dpsi", "sc"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    if prec == "fp64":
+        monkeypatch.setenv("DMC_HPSI_TABLES", "lds")
+        e, s = engines(n, norb, nw, k, precision, dcut=2.0)
+        x, irn = s.init_walkers(5, nw)
+        dpsi, sc = s.init_hpsi(x)
+        e.push_walkers(x, irn)
+        X, D, SC, R, nacc, _ = oracle_step(s, x, dpsi, sc, irn, k)
+        e.step()
+        w = e.walkers()
+        go = np.argsort(w["irn"])
+        oo = np.argsort(R)
+        np.testing.assert_array_equal(w["irn"][go], R[oo])
+        np.testing.assert_allclose(w["x"][go], X[oo], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(w["psil"][go], SC[oo, 1], rtol=1e-10)
+        e.close()
