@@ -195,62 +195,88 @@ def precision_study(nconf=4096, ks=(N, 33, 16), nequil=200):
     return out
 
 
-def _sample_energy(e, nequil, nblocks, blocklen):
-    """block means of the weighted energy estimator E_L / (N E_FG) after nequil steps: their
-    mean and standard error, and the number of blocks left out as outliers.  A block is an
-    outlier when its mean lies more than 10 robust standard deviations (1.4826 times the
-    median absolute deviation) from the median block: one walker whose local energy spikes
-    (a pair landing next to a node or in the narrow potential well) carries an uncapped
-    estimator weight exp(-(E_L - E_T) dt) (step.f90:137-156) that decides its block alone."""
+def _energy_series(e, nequil, nmeas):
+    """the population's energy estimator E_L / (N E_FG) (estimator 1, step.f90:137-156) of
+    every step after nequil steps"""
     for _ in range(nequil):
         e.step()
     e.estimators(reset=True)
-    blocks = []
-    for _ in range(nblocks):
-        for _ in range(blocklen):
-            e.step(opcomp=True)
+    x = np.empty(nmeas)
+    for i in range(nmeas):
+        e.step(opcomp=True)
         est = e.estimators(reset=True)
-        blocks.append(est[0, 0] / est[0, 2])
-    b = np.array(blocks)
-    med = np.median(b[np.isfinite(b)]) if np.isfinite(b).any() else 0.0
-    mad = 1.4826 * np.median(np.abs(b[np.isfinite(b)] - med)) if np.isfinite(b).any() else 0.0
-    keep = np.isfinite(b) & (np.abs(b - med) <= 10.0 * max(mad, 1e-12))
-    k = b[keep]
-    return (float(k.mean()), float(k.std(ddof=1) / np.sqrt(len(k))), int((~keep).sum()))
+        x[i] = est[0, 0] / est[0, 2]
+    return x
 
 
-def energy_study(nequil=2000, nblocks=20, blocklen=50):
+def _series_mean(x):
+    """mean and standard error of a series of per-step energies.
+
+    Steps whose energy lies more than 10 robust standard deviations (1.4826 times the median
+    absolute deviation) from the median are left out: in such a step one walker whose local
+    energy spikes (a pair in the narrow potential well, or next to a node) carries an uncapped
+    estimator weight exp(-(E_L - E_T) dt) (step.f90:137-156) and decides the step alone.
+    Successive steps of one population are correlated, so the error is sigma sqrt(2 tau_int
+    / n) with the integrated autocorrelation time tau_int summed up to the first lag
+    M >= 5 tau_int (the automatic window of Madras and Sokal, J. Stat. Phys. 50 (1988) 109).
+    Returns (mean, error, tau_int in steps, steps left out)."""
+    med = float(np.median(x))
+    mad = 1.4826 * float(np.median(np.abs(x - med)))
+    keep = np.isfinite(x) & (np.abs(x - med) <= 10.0 * max(mad, 1e-12))
+    y = x[keep]
+    n = len(y)
+    d = y - y.mean()
+    f = np.fft.rfft(d, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    acf = acf / acf[0] if acf[0] > 0 else np.zeros(n)
+    tau = 0.5
+    for m in range(1, n):
+        tau += float(acf[m])
+        if m >= 5.0 * tau:
+            break
+    se = float(y.std(ddof=1) * np.sqrt(2.0 * tau / n))
+    return float(y.mean()), se, tau, int((~keep).sum())
+
+
+def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000):
     """DMC energies of the headline system from an equilibrated population.
 
-    One population (32768 walkers) is equilibrated by nequil mixed-precision steps at tau.
-    From it: the mixed and the FP64 single-particle propagators at tau (same walkers and
-    random streams, independent chains after a few steps), and the reference's own
-    algorithm, the all-particle step (the device replay that reproduces the reference's
-    trajectories), at tau after 500 further steps; then the FP64 single-particle and the
-    all-particle propagators at tau/2 (1000 steps of re-equilibration, blocks of twice the
-    steps).  Every propagator cuts the drift at 2 sigma (the reference's dcut,
-    step.f90:32-37): with the deck's dcut = -1 the all-particle step from an equilibrated
-    population lets walkers next to a node overshoot into regions of very negative local
-    energy whose uncapped weights exp(-(E_L - E_T) dt) swamp the estimator (E = -132 +- 132
-    in one such run).  Each energy is the mean of nblocks block means (estimator 1 every
-    step) with its standard error; E(0) = 2 E(tau/2) - E(tau) removes the linear time-step
-    error of each algorithm."""
+    One population (32768 walkers) is equilibrated by nequil mixed-precision steps at tau
+    from the reference's initial walkers (copies of one configuration; their energy settles
+    after about 4.5 units of imaginary time).  From it:
+    * the mixed and the FP64 single-particle sweeps at tau (same walkers and random streams;
+      the FP64 run is nfp64 steps, compared with the mixed run's first nfp64 steps), and the
+      mixed sweep at tau/2 (nsm steps each, tau/2 after 1000 steps of re-equilibration);
+    * the reference's own algorithm, the all-particle step (the device replay that reproduces
+      the reference's trajectories), at tau, tau/2 and tau/4: nap f steps after 1000 f steps
+      (one unit of imaginary time) of re-equilibration (f = 1, 2, 4).  Its time-step error is large and not linear at these
+      steps, so E(0) comes from the quadratic through the three, (8 E(tau/4) - 6 E(tau/2) +
+      E(tau)) / 3.
+    Every propagator cuts the drift at 2 sigma (the reference's dcut, step.f90:32-37): with
+    the deck's dcut = -1 the all-particle step from an equilibrated population lets walkers
+    next to a node overshoot into regions of very negative local energy whose uncapped
+    weights swamp the estimator (E = -132 +- 132 in one such run).  Energies and errors per
+    run come from the per-step energies (_series_mean)."""
     e = make_engine("mixed")
     for _ in range(nequil):
         e.step()
     w = e.walkers()
     walkers = (w["x"], w["irn"])
     e.close()
-    runs = {}
-    for label, mode, dt, neq, blen in (("mixed", "mixed", DT, 0, blocklen),
-                                       ("fp64", "fp64", DT, 0, blocklen),
-                                       ("allparticle", "allparticle", DT, 500, blocklen),
-                                       ("fp64_half_tau", "fp64", DT / 2, 1000, 2 * blocklen),
-                                       ("allparticle_half_tau", "allparticle", DT / 2, 1000,
-                                        2 * blocklen)):
+    series, runs = {}, {}
+    for label, mode, dt, neq, nmeas in (("mixed", "mixed", DT, 0, nsm),
+                                         ("fp64", "fp64", DT, 0, nfp64),
+                                         ("mixed_half_tau", "mixed", DT / 2, 1000, nsm),
+                                         ("allparticle", "allparticle", DT, 1000, nap),
+                                         ("allparticle_half_tau", "allparticle", DT / 2, 2000,
+                                          2 * nap),
+                                         ("allparticle_quarter_tau", "allparticle", DT / 4, 4000,
+                                          4 * nap)):
         e = make_engine(mode, dt=dt, walkers=walkers, dcut=DCUT_SM)
-        runs[label] = _sample_energy(e, neq, nblocks, blen)
+        series[label] = _energy_series(e, neq, nmeas)
         e.close()
+        m, se, tau, nout = _series_mean(series[label])
+        runs[label] = dict(e=m, err=se, steps=nmeas, tau_int=tau, outlier_steps=nout)
         print(f"bench: energy {label}: {runs[label]}", file=sys.stderr, flush=True)
 
     def diff(a, b):
@@ -258,22 +284,26 @@ def energy_study(nequil=2000, nblocks=20, blocklen=50):
         sig = float(np.hypot(a[1], b[1]))
         return dict(diff=d, sigma=sig, diff_over_sigma=d / sig if sig > 0 else None)
 
-    def extrap(full, half):
-        return (2 * half[0] - full[0], float(np.hypot(2 * half[1], full[1])))
+    def ev(label):
+        return runs[label]["e"], runs[label]["err"]
 
-    e0_sm = extrap(runs["fp64"], runs["fp64_half_tau"])
-    e0_ap = extrap(runs["allparticle"], runs["allparticle_half_tau"])
+    m1 = _series_mean(series["mixed"][:nfp64])
+    e1, e2, e4 = ev("allparticle"), ev("allparticle_half_tau"), ev("allparticle_quarter_tau")
+    ap0 = ((8 * e4[0] - 6 * e2[0] + e1[0]) / 3,
+           float(np.sqrt((8 * e4[1]) ** 2 + (6 * e2[1]) ** 2 + e1[1] ** 2) / 3))
+    ap0_lin = (2 * e4[0] - e2[0], float(np.hypot(2 * e4[1], e2[1])))
+    sm = ev("mixed_half_tau")
     return dict(unit="E_L / (N E_FG)", tau=DT,
                 start=f"{nequil} mixed steps from the reference's initial walkers",
-                blocks=f"{nblocks} blocks of {blocklen} steps at tau, {2 * blocklen} at tau/2",
-                runs={k: dict(e=v[0], err=v[1], outlier_blocks=v[2]) for k, v in runs.items()},
-                mixed_vs_fp64=diff(runs["mixed"], runs["fp64"]),
-                singleparticle_vs_allparticle_tau=diff(runs["fp64"], runs["allparticle"]),
-                singleparticle_vs_allparticle_half_tau=diff(runs["fp64_half_tau"],
-                                                            runs["allparticle_half_tau"]),
-                extrapolated_singleparticle=dict(e=e0_sm[0], err=e0_sm[1]),
-                extrapolated_allparticle=dict(e=e0_ap[0], err=e0_ap[1]),
-                extrapolated_diff=diff(e0_sm, e0_ap))
+                error="per-step energies, integrated autocorrelation time (automatic "
+                      "window); steps beyond 10 robust sigma left out",
+                runs=runs,
+                mixed_vs_fp64=dict(diff(m1, ev("fp64")), steps=nfp64),
+                singleparticle_tau_vs_half_tau=diff(ev("mixed"), sm),
+                allparticle_extrapolated=dict(e=ap0[0], err=ap0[1], fit="quadratic in tau"),
+                allparticle_extrapolated_linear=dict(e=ap0_lin[0], err=ap0_lin[1],
+                                                     fit="tau/2, tau/4"),
+                singleparticle_vs_allparticle_extrapolated=diff(sm, ap0))
 
 
 BATCH_SYSTEMS = [(14, 3, 8192), (38, 4, 16384), (66, 5, 24576)]
