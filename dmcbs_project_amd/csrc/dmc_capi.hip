@@ -203,6 +203,10 @@ int upload_tables(dmc_ctx *c) {
     c->ds.tab_lds = 0;
     const char *tv = getenv("DMC_HPSI_TABLES");
     if (c->ds.keep_sinv && tv && !strcmp(tv, "lds")) c->ds.tab_lds = sm_end_tab_fits(s.n, c->ds);
+    c->ds.mv_lds = 0;
+    const char *mv = getenv("DMC_MOVE_TABLES");
+    if (c->ds.keep_sinv && c->ds.sinv_f32 && mv && !strcmp(mv, "lds"))
+        c->ds.mv_lds = sm_moves_tab_fits(s.n, c->ds);
     return 0;
 }
 

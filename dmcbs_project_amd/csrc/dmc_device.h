@@ -23,6 +23,9 @@ struct DevSys {
     // the end-of-sweep kernel stages the radial tables in the LDS of workgroups of several
     // walkers (1), or reads them from global memory with one walker per workgroup (0)
     int tab_lds;
+    // the two-walker FP32 move kernel stages its pair records in the LDS of workgroups of
+    // several waves (1), or reads them from global memory, one wave per workgroup (0)
+    int mv_lds;
     const double4 *bt;    // beta records {b, db, lap b, 0}, [ntab+1]
     // the same beta values in compact arrays, so that a pair loop touches only the fields it
     // reads (a smaller working set in the vector L1): b, [ntab+1]; {db, lap b}, [ntab+1]

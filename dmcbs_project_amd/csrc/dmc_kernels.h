@@ -35,6 +35,8 @@ int launch_sm_end(int nup, const DevSys &S, const DevState &A, const DevState &B
                   const DevStep &T, int opcomp, int grid, hipStream_t st);
 // 1 if a workgroup of the end kernel that stages the radial tables in LDS fits the device
 int sm_end_tab_fits(int nup, const DevSys &S);
+// 1 if a workgroup of the two-walker move kernel that stages its pair records in LDS fits
+int sm_moves_tab_fits(int nup, const DevSys &S);
 // parameter adjoint of nw configurations: d log psi and d E_L per parameter, [nw][norb + 2]
 int launch_adjoint(int nup, const DevSys &S, const double *x, int nw, int norb, double *dlogpsi,
                    double *delocal, hipStream_t st);

@@ -67,6 +67,27 @@ __global__ __launch_bounds__(64) void k_sm_moves32x2(DevSys S, const double *__r
     sm_moves_pair<NUP>(S, sm, wp, nw, threadIdx.x, xin, xout, irn_in, irn_out, sinv_g, m0, m1, T);
 }
 
+// The same moves with the FP32 pair records staged in LDS: workgroups of kMvWaves waves (two
+// walkers each) copy the 16 KB table once, then each wave moves its pair of walkers.
+template <int NUP>
+__global__ __launch_bounds__(64 * kMvWaves) void k_sm_moves32x2_tab(
+    DevSys S, const double *__restrict__ xin, double *xout, const uint64_t *irn_in,
+    uint64_t *irn_out, char *sinv_g, int m0, int m1, DevStep T) {
+    extern __shared__ double smem[];
+    const int n1 = S.ntab + 1;
+    float4 *tab = reinterpret_cast<float4 *>(smem);
+    for (int i = threadIdx.x; i < n1; i += 64 * kMvWaves) tab[i] = S.mt32[i];
+    __syncthreads();
+    const int nw = T.nt[0];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wp = blockIdx.x * kMvWaves + wave;
+    if (2 * wp >= nw) return;
+    double *sm = smem + 2 * n1 + wave * PairLayout<NUP>::TOTAL;
+    const MtL mt{(lds_cv4f *)(smem)};
+    sm_moves_pair<NUP>(S, mt, sm, wp, nw, threadIdx.x & 63, xin, xout, irn_in, irn_out, sinv_g,
+                       m0, m1, T);
+}
+
 // FP64 rebuild of the inverse of positions x (every krefresh moves)
 template <int NUP>
 __global__ __launch_bounds__(64) void k_sm_refresh(DevSys S, const double *__restrict__ x,
@@ -132,9 +153,16 @@ int sm_moves_t(const DevSys &S, const double *xin, double *xout, const uint64_t 
     if (S.nq > 64) return hipErrorInvalidValue;  // four projection chunks of 32
     if constexpr (NUP >= 17) {
         if (S.sinv_f32 && mv_pair_fits(NUP, S.nq)) {
-            hipLaunchKernelGGL(k_sm_moves32x2<NUP>, dim3((grid + 1) / 2), dim3(64),
-                               mv_pair_lds_bytes<NUP>(), st, S, xin, xout, irn_in, irn_out,
-                               sinv, m0, m1, T);
+            const int pairs = (grid + 1) / 2;
+            if (S.mv_lds)
+                hipLaunchKernelGGL(k_sm_moves32x2_tab<NUP>,
+                                   dim3((pairs + kMvWaves - 1) / kMvWaves), dim3(64 * kMvWaves),
+                                   mv_pair_tab_lds_bytes<NUP>(S.ntab), st, S, xin, xout, irn_in,
+                                   irn_out, sinv, m0, m1, T);
+            else
+                hipLaunchKernelGGL(k_sm_moves32x2<NUP>, dim3(pairs), dim3(64),
+                                   mv_pair_lds_bytes<NUP>(), st, S, xin, xout, irn_in, irn_out,
+                                   sinv, m0, m1, T);
             return hipGetLastError();
         }
     }
@@ -190,6 +218,27 @@ int launch_sm_moves(int nup, const DevSys &S, const double *xin, double *xout,
         case 33: return sm_moves_t<33>(S, xin, xout, irn_in, irn_out, sinv, m0, m1, T, grid, st);
     }
     return hipErrorInvalidValue;
+}
+
+template <int NUP>
+int sm_moves_tab_fits_t(const DevSys &S) {
+    if constexpr (NUP >= 17) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void *>(k_sm_moves32x2_tab<NUP>), 64 * kMvWaves,
+                mv_pair_tab_lds_bytes<NUP>(S.ntab)) != hipSuccess)
+            return 0;
+        return nb > 0;
+    }
+    return 0;
+}
+
+int sm_moves_tab_fits(int nup, const DevSys &S) {
+    switch (nup) {
+        case 19: return sm_moves_tab_fits_t<19>(S);
+        case 33: return sm_moves_tab_fits_t<33>(S);
+    }
+    return 0;
 }
 
 int sm_end_tab_fits(int nup, const DevSys &S) {

@@ -155,11 +155,29 @@ __device__ __forceinline__ void partner_phases(const DevSys &S, const double *xs
     }
 }
 
+// Views of the moves' FP32 pair records {b, db, u, du} [ntab+1]: MtG reads them from global
+// memory (the system's copy, through the vector L1), MtL from a copy staged in the LDS of a
+// workgroup of several waves.
+struct MtG {
+    const float4 *t;
+    __device__ float4 at(int i) const { return t[i]; }
+};
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const v4f lds_cv4f;
+struct MtL {
+    lds_cv4f *t;
+    __device__ float4 at(int i) const {
+        const v4f v = t[i];
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+};
+
 // Pair terms of the mover at pa with a partner at pb (both FP64 positions in registers), as
 // pair_terms of dmc_sm.h but without a branch: a pair beyond the table range (about half of
 // all pairs of a uniform walker, scattered over the lanes) reads the last interval and takes
 // zero weights, so that the lanes never split and the pair passes stay in one block.
-__device__ __forceinline__ void pair_terms_v(const DevSys &S, const double (&pa)[3],
+template <class MT>
+__device__ __forceinline__ void pair_terms_v(const DevSys &S, const MT &mt, const double (&pa)[3],
                                              const double (&pb)[3], float (&dx)[3], float &tb,
                                              float &tj, float &u, float &bv) {
     const float r = pair_dist<float>(S, pa, pb, dx);
@@ -176,7 +194,7 @@ __device__ __forceinline__ void pair_terms_v(const DevSys &S, const double (&pa)
     w1 *= z;
     w2 *= z;
     w3 *= z;
-    const float4 a = S.mt32[idx - 1], b = S.mt32[idx], c = S.mt32[idx + 1], d = S.mt32[idx + 2];
+    const float4 a = mt.at(idx - 1), b = mt.at(idx), c = mt.at(idx + 1), d = mt.at(idx + 2);
     bv = w0 * a.x + w1 * b.x + w2 * c.x + w3 * d.x;
     const float db = w0 * a.y + w1 * b.y + w2 * c.y + w3 * d.y;
     u = w0 * a.z + w1 * b.z + w2 * c.z + w3 * d.z;
@@ -188,9 +206,9 @@ __device__ __forceinline__ void pair_terms_v(const DevSys &S, const double (&pa)
 
 // Moves m0..m1-1 of walkers 2 wp and 2 wp + 1 (nw walkers in all; a half whose walker does
 // not exist repeats the last walker and writes nothing)
-template <int NUP>
-__device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int wp, int nw,
-                                              int tid, const double *__restrict__ xin,
+template <int NUP, class MT>
+__device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, double *sm, int wp,
+                                              int nw, int tid, const double *__restrict__ xin,
                                               double *xout, const uint64_t *irn_in,
                                               uint64_t *irn_out, char *sinv_g, int m0, int m1,
                                               const DevStep &T) {
@@ -279,8 +297,8 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
                 q32[d] = xs[3 * (p0 + 32) + d];
             }
             float bv;
-            pair_terms_v(S, pa, q31, e1d, e1b, e1j, e1u, bv);
-            pair_terms_v(S, pa, q32, e2d, e2b, e2j, e2u, bv);
+            pair_terms_v(S, mt, pa, q31, e1d, e1b, e1j, e1u, bv);
+            pair_terms_v(S, mt, pa, q32, e2d, e2b, e2j, e2u, bv);
         }
     };
     setup_half();
@@ -338,7 +356,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
 #pragma unroll
                     for (int d = 0; d < 3; ++d) q3[d] = xs[3 * (p0 + 31 + k) + d];
                     float d3[3], b3, j3, u3, bv;
-                    pair_terms_v(S, xa, q3, d3, b3, j3, u3, bv);
+                    pair_terms_v(S, mt, xa, q3, d3, b3, j3, u3, bv);
                     if (k == 0) {
                         e1b = b3; e1j = j3; e1u = u3;
 #pragma unroll
@@ -354,7 +372,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
         if (!have_old) {
             if (k < NB) {
                 float bv;
-                pair_terms_v(S, xa, xp, odx, ob, oj, ou, bv);
+                pair_terms_v(S, mt, xa, xp, odx, ob, oj, ou, bv);
             }
             float sn, cs;
             sincos_2pi<float>((cm.x * xa[0] + cm.y * xa[1] + cm.z * xa[2]) * S.eli, sn, cs);
@@ -434,7 +452,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
         // of w_q exp(i q.x'): their table loads are then in flight while the phases are formed.
         // Pass A: the proposed position with partner k (lanes < NA).
         float adx[3] = {0, 0, 0}, atb = 0, atj = 0, auu = 0, abv = 0;
-        if (k < NA) pair_terms_v(S, xn, xp, adx, atb, atj, auu, abv);
+        if (k < NA) pair_terms_v(S, mt, xn, xp, adx, atb, atj, auu, abv);
         const float ou_cur = ou;  // the mover's old u with partner k (pass B overwrites it)
         // Pass B: the next mover of this spin at its current position (lanes < NB), and with
         // n = 33 the proposed position with partner 32 (lane 31)
@@ -448,7 +466,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
                 pb[d] = lb ? xs[3 * (p0 + 32 < 2 * NUP ? p0 + 32 : 0) + d] : xp[d];
             }
             float d3[3], b3, j3, u3, bv;
-            pair_terms_v(S, pa, pb, d3, b3, j3, u3, bv);
+            pair_terms_v(S, mt, pa, pb, d3, b3, j3, u3, bv);
             const bool keep = nxt && k < NB;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -606,6 +624,25 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int w
         atomicAdd(reinterpret_cast<unsigned long long *>(&T.moves[1]),
                   (unsigned long long)((m1 - m0) * (both ? 2 : 1)));
     }
+}
+
+template <int NUP>
+__device__ __forceinline__ void sm_moves_pair(const DevSys &S, double *sm, int wp, int nw,
+                                              int tid, const double *__restrict__ xin,
+                                              double *xout, const uint64_t *irn_in,
+                                              uint64_t *irn_out, char *sinv_g, int m0, int m1,
+                                              const DevStep &T) {
+    sm_moves_pair<NUP>(S, MtG{S.mt32}, sm, wp, nw, tid, xin, xout, irn_in, irn_out, sinv_g, m0,
+                       m1, T);
+}
+
+// waves (walker pairs) per workgroup of the move kernel that stages the pair records in LDS
+constexpr int kMvWaves = 4;
+
+// its LDS: the pair records [ntab+1] (16-byte aligned), then one two-walker area per wave
+template <int NUP>
+size_t mv_pair_tab_lds_bytes(int ntab) {
+    return size_t(ntab + 1) * sizeof(float4) + kMvWaves * mv_pair_lds_bytes<NUP>();
 }
 
 }  // namespace dmc
