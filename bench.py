@@ -332,7 +332,8 @@ def batch_run(steps, warmup, tune=True):
     if tune:
         choice, ms = b.tune(reps=3)
         res["wpg"] = choice
-        # ms per launch at 1, 2, 4, 8 walkers per workgroup and per-system launches (0)
+        # ms per launch at 1, 2, 4, 8 walkers per workgroup and per-system launches with the
+        # tables read from global memory (0) or staged in LDS (-1)
         res["tune_ms"] = {k: [round(float(v), 4) for v in row]
                           for k, row in zip(("moves", "refresh", "end"), ms)}
     for _ in range(warmup):

@@ -354,10 +354,11 @@ class Batch:
 
     def tune(self, reps=3):
         """Launch setting of every phase from timed launches; returns the choice (walkers
-        per workgroup, 0 = one launch per system) and the ms per launch
-        [phase][setting = 1, 2, 4, 8, 0] (-1: not run)."""
+        per workgroup, 0 = one launch per system, -1 = one launch per system of the kernels
+        that stage their tables in LDS) and the ms per launch
+        [phase][setting = 1, 2, 4, 8, 0, -1] (-1 ms: not run)."""
         w = np.zeros(3, np.int32)
-        ms = np.zeros(15)
+        ms = np.zeros(18)
         L.check(self.lib.dmc_batch_tune(self.h, int(reps), w, ms))
-        return dict(zip(("moves", "refresh", "end"), (int(v) for v in w))), ms.reshape(3, 5)
+        return dict(zip(("moves", "refresh", "end"), (int(v) for v in w))), ms.reshape(3, 6)
 

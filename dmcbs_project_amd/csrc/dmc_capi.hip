@@ -1148,9 +1148,20 @@ void batch_drop_graphs(dmc_batch *b) {
     b->dirty = true;
 }
 
+// The system as the per-system launches of a phase see it: setting 0 keeps the context's
+// kernels, setting -1 selects the kernels that stage their tables in LDS (the two-walker
+// moves, the end of the sweep) where their workgroups fit
+DevSys phase_sys(const dmc_ctx *c, int nup, BatchPhase ph, int setting) {
+    DevSys d = c->ds;
+    if (setting < 0 && ph == BatchPhase::Moves && d.sinv_f32) d.mv_lds = sm_moves_tab_fits(nup, d);
+    if (setting < 0 && ph == BatchPhase::End) d.tab_lds = sm_end_tab_fits(nup, d);
+    return d;
+}
+
 // One phase of a batch step.  setting = walkers per workgroup of the one persistent launch
-// over all systems, or 0: one launch of the single-system kernel per system (grid = its
-// capacity; workgroups past the device population return at once).
+// over all systems, or one launch of the single-system kernel per system (grid = its
+// capacity; workgroups past the device population return at once): 0 with the context's
+// kernels, -1 with the kernels that stage their tables in LDS.
 int batch_phase(dmc_batch *b, BatchPhase ph, int chunk, int opcomp, int *counter, int setting) {
     if (setting > 0)
         return launch_batch_phase(b->host, b->args, ph, chunk, opcomp, counter, setting, b->ncu,
@@ -1158,17 +1169,18 @@ int batch_phase(dmc_batch *b, BatchPhase ph, int chunk, int opcomp, int *counter
     for (size_t i = 0; i < b->order.size(); ++i) {
         dmc_ctx *c = b->ctx[b->order[i]];
         const BatchSys &q = b->host.sys[i];
+        const DevSys d = phase_sys(c, q.nup, ph, setting);
         int e = 0;
         if (ph == BatchPhase::End) {
-            e = launch_sm_end(q.nup, c->ds, c->A, c->B, c->T, opcomp, c->capacity, b->st);
+            e = launch_sm_end(q.nup, d, c->A, c->B, c->T, opcomp, c->capacity, b->st);
         } else {
             const int m0 = chunk * q.k;
             if (m0 >= q.npart) continue;
             const int m1 = m0 + q.k < q.npart ? m0 + q.k : q.npart;
             if (ph == BatchPhase::Refresh)
-                e = launch_sm_refresh(q.nup, c->ds, c->B.x, c->A.sinv, c->T, c->capacity, b->st);
+                e = launch_sm_refresh(q.nup, d, c->B.x, c->A.sinv, c->T, c->capacity, b->st);
             else
-                e = launch_sm_moves(q.nup, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
+                e = launch_sm_moves(q.nup, d, m0 ? c->B.x : c->A.x, c->B.x,
                                     m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1, c->T,
                                     c->capacity, b->st);
         }
@@ -1183,17 +1195,19 @@ int batch_phase(dmc_batch *b, BatchPhase ph, int chunk, int opcomp, int *counter
 int batch_system_chain(dmc_batch *b, int i, int opcomp, hipStream_t s) {
     dmc_ctx *c = b->ctx[b->order[i]];
     const BatchSys &q = b->host.sys[i];
+    const DevSys dm = phase_sys(c, q.nup, BatchPhase::Moves, b->wpg[0]);
+    const DevSys de = phase_sys(c, q.nup, BatchPhase::End, b->wpg[2]);
     for (int ch = 0; ch < b->nchunk; ++ch) {
         const int m0 = ch * q.k;
         if (m0 >= q.npart) break;
         const int m1 = m0 + q.k < q.npart ? m0 + q.k : q.npart;
         if (ch > 0)
             LAUNCHCHK(launch_sm_refresh(q.nup, c->ds, c->B.x, c->A.sinv, c->T, c->capacity, s));
-        LAUNCHCHK(launch_sm_moves(q.nup, c->ds, m0 ? c->B.x : c->A.x, c->B.x,
+        LAUNCHCHK(launch_sm_moves(q.nup, dm, m0 ? c->B.x : c->A.x, c->B.x,
                                   m0 ? c->B.irn : c->A.irn, c->B.irn, c->A.sinv, m0, m1, c->T,
                                   c->capacity, s));
     }
-    LAUNCHCHK(launch_sm_end(q.nup, c->ds, c->A, c->B, c->T, opcomp, c->capacity, s));
+    LAUNCHCHK(launch_sm_end(q.nup, de, c->A, c->B, c->T, opcomp, c->capacity, s));
     LAUNCHCHK(launch_est_reduce(c->T, -1, c->est_acc, s));
     LAUNCHCHK(launch_branch_device(c->ds, c->A, c->B, c->T, c->capacity, s));
     return DMC_OK;
@@ -1203,7 +1217,7 @@ int batch_system_chain(dmc_batch *b, int i, int opcomp, hipStream_t s) {
 int batch_enqueue(dmc_batch *b, int opcomp) {
     const int ns = int(b->order.size());
     LAUNCHCHK(launch_batch_begin(b->args, b->counters, dmc_batch::kBatchCounters, b->st));
-    if (b->wpg[0] == 0 && b->wpg[2] == 0 && (b->nchunk == 1 || b->wpg[1] == 0)) {
+    if (b->wpg[0] <= 0 && b->wpg[2] <= 0 && (b->nchunk == 1 || b->wpg[1] <= 0)) {
         // no fused launch: the systems share nothing, so their chains run side by side
         HIPCHK(hipEventRecord(b->fork, b->st));
         for (int i = 0; i < ns; ++i) {
@@ -1374,8 +1388,11 @@ int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end)
     const int32_t w[3] = {moves, refresh, end};
     const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
     for (int i = 0; i < 3; ++i) {
-        if (w[i] != 0 && w[i] != 1 && w[i] != 2 && w[i] != 4 && w[i] != 8)
-            return fail(DMC_ERR_ARG, "walkers per workgroup must be 1, 2, 4 or 8 (0: per system)");
+        if (w[i] != -1 && w[i] != 0 && w[i] != 1 && w[i] != 2 && w[i] != 4 && w[i] != 8)
+            return fail(DMC_ERR_ARG, "walkers per workgroup must be 1, 2, 4 or 8 (0: per "
+                                     "system, -1: per system with LDS-staged tables)");
+        if (w[i] == -1 && ph[i] == BatchPhase::Refresh)
+            return fail(DMC_ERR_ARG, "the refresh has no LDS-table kernel (-1)");
         if (w[i] > 0 && batch_blocks_per_cu(b->host, ph[i], w[i]) <= 0)
             return fail(DMC_ERR_ARG, "walkers per workgroup " + std::to_string(w[i]) +
                                          " does not fit the LDS or registers of a CU");
@@ -1477,8 +1494,8 @@ int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out)
         b->args.sys[i] = b->host.sys[i];
     }
     const BatchPhase ph[3] = {BatchPhase::Moves, BatchPhase::Refresh, BatchPhase::End};
-    constexpr int kCand = 5;
-    const int cand[kCand] = {1, 2, 4, 8, 0};
+    constexpr int kCand = 6;
+    const int cand[kCand] = {1, 2, 4, 8, 0, -1};
     for (int p = 0; p < 3; ++p) {
         if (p == 1 && b->nchunk < 2) {
             if (ms_out) for (int k = 0; k < kCand; ++k) ms_out[kCand * p + k] = -1.0;
@@ -1488,7 +1505,11 @@ int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out)
         int bw = b->wpg[p];
         for (int k = 0; k < kCand; ++k) {
             double t = -1.0;
-            if (cand[k] == 0 || batch_blocks_per_cu(b->host, ph[p], cand[k]) > 0) {
+            // the LDS-table setting exists for the moves and the end of the sweep
+            const bool run = cand[k] == 0 || (cand[k] < 0 ? ph[p] != BatchPhase::Refresh
+                                                          : batch_blocks_per_cu(b->host, ph[p],
+                                                                                cand[k]) > 0);
+            if (run) {
                 const int chunk = p == 2 ? -1 : (p == 1 ? 1 : 0);
                 float ms = 0.f;
                 for (int r = 0; r <= reps && !e; ++r) {  // the first launch warms up

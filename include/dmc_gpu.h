@@ -234,14 +234,17 @@ int dmc_batch_step(dmc_batch *b, int32_t opcomp);
 int dmc_batch_run(dmc_batch *b, int32_t nsteps);  /* opcomp on the last step */
 int dmc_batch_sync(dmc_batch *b, int32_t *nt);
 /* Launch setting of the three phases: walkers per workgroup (1, 2, 4 or 8; one walker per
- * 64-lane wave) of the one persistent launch over all systems, or 0 for one launch of the
- * single-system kernel per system (all inside the step graph; when no phase uses the
- * persistent launch, each system's launches run on a stream of their own inside the graph,
- * side by side). */
+ * 64-lane wave) of the one persistent launch over all systems, or one launch of the
+ * single-system kernel per system: 0 with the kernels that read their tables from global
+ * memory, -1 (moves and end of the sweep) with the kernels that stage their tables in the LDS
+ * of workgroups of several walkers.  All launches are inside the step graph; when no phase
+ * uses the persistent launch, each system's launches run on a stream of their own inside the
+ * graph, side by side. */
 int dmc_batch_set_wpg(dmc_batch *b, int32_t moves, int32_t refresh, int32_t end);
-/* Autotuner: times reps launches of every phase at the settings 1, 2, 4, 8 and 0 on the
- * resident populations (state left unchanged) and keeps the fastest.  wpg_out[3] = the
- * choice per phase; ms_out[3][5] = ms per launch (-1: does not fit or phase unused). */
+/* Autotuner: times reps launches of every phase at the settings 1, 2, 4, 8, 0 and -1 on the
+ * resident populations (state left unchanged) and keeps the fastest: walkers per workgroup
+ * and where the tables sit (L1 or LDS).  wpg_out[3] = the choice per phase; ms_out[3][6] =
+ * ms per launch (-1: does not fit, or the phase has no such setting or is unused). */
 int dmc_batch_tune(dmc_batch *b, int32_t reps, int32_t *wpg_out, double *ms_out);
 
 #ifdef __cplusplus

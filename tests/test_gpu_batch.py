@@ -83,10 +83,10 @@ def test_batch_walkers_per_workgroup_and_tuner():
         e.step()
         e.step()
     results = []
-    for wpg in (1, 2, 4, 0):
+    for wpg in (1, 2, 4, 0, -1):
         bat = engines(DMC_MIXED)
         b = Batch(bat)
-        b.set_wpg(wpg, 1 if wpg else 0, min(wpg, 2))
+        b.set_wpg(wpg, 1 if wpg > 0 else 0, min(wpg, 2))
         b.step()
         b.step()
         b.sync()
@@ -97,8 +97,9 @@ def test_batch_walkers_per_workgroup_and_tuner():
     b = Batch(bat)
     choice, ms = b.tune(reps=2)
     assert set(choice) == {"moves", "refresh", "end"}
-    assert all(v in (0, 1, 2, 4, 8) for v in choice.values())
-    assert (ms[0] > 0).any() and (ms[2] > 0).any()
+    assert all(v in (-1, 0, 1, 2, 4, 8) for v in choice.values())
+    assert choice["refresh"] != -1 and ms[1, 5] == -1
+    assert (ms[0] > 0).any() and (ms[2] > 0).any() and ms[0, 5] > 0 and ms[2, 5] > 0
     b.step()
     b.step()
     b.sync()
