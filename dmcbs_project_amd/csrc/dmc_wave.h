@@ -505,48 +505,16 @@ struct Tile {
     }
 };
 
-// Argmax of (|pivot| candidate, row) over the lanes that share lane & 7 (the 8 lanes of a
-// tile column group), ties to the lower row; every lane of the group ends with the result.
-// The exchanges are VALU permutes: DPP row_ror:8 (lane ^ 8) and the permlane16/32 swaps
-// (lane ^ 16, lane ^ 32).  A swap of a register with itself returns the same ordered pair of
-// values in both lanes of a partner pair, and the xor-8 pair is put in lane order, so that the
-// choice is made identically in both lanes (NaN candidates included).
-__device__ __forceinline__ void pick_pivot(double b1, int r1, double b2, int r2, double &b,
-                                           int &r) {
-    const bool t = b2 > b1 || (b2 == b1 && r2 < r1);
-    b = t ? b2 : b1;
-    r = t ? r2 : r1;
-}
-__device__ __forceinline__ void pivot_argmax(double &best, int &brow, int tid) {
-    {
-        const long long bb = __double_as_longlong(best);
-        const int lo = __builtin_amdgcn_update_dpp(0, int(bb & 0xffffffffll), 0x128, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, int(bb >> 32), 0x128, 0xF, 0xF, false);
-        const int orow = __builtin_amdgcn_update_dpp(0, brow, 0x128, 0xF, 0xF, false);
-        const double ob = dbl_of(lo, hi);
-        if (tid & 8) pick_pivot(ob, orow, best, brow, best, brow);
-        else pick_pivot(best, brow, ob, orow, best, brow);
-    }
-    {
-        const long long bb = __double_as_longlong(best);
-        const unsigned lo = unsigned(bb & 0xffffffffll), hi = unsigned(bb >> 32);
-        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-        const auto rr = __builtin_amdgcn_permlane16_swap(unsigned(brow), unsigned(brow), false,
-                                                         false);
-        pick_pivot(dbl_of(int(l[0]), int(h[0])), int(rr[0]), dbl_of(int(l[1]), int(h[1])),
-                   int(rr[1]), best, brow);
-    }
-    {
-        const long long bb = __double_as_longlong(best);
-        const unsigned lo = unsigned(bb & 0xffffffffll), hi = unsigned(bb >> 32);
-        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-        const auto rr = __builtin_amdgcn_permlane32_swap(unsigned(brow), unsigned(brow), false,
-                                                         false);
-        pick_pivot(dbl_of(int(l[0]), int(h[0])), int(rr[0]), dbl_of(int(l[1]), int(h[1])),
-                   int(rr[1]), best, brow);
-    }
+// Largest 32-bit key over the lanes that share lane & 7 (the 8 lanes of a tile column group):
+// integer maxima with the lanes ^ 8 (DPP row_ror:8), ^ 16 and ^ 32 (permlane swaps of a
+// register with itself, which give both lanes of a pair the same two values); every lane of
+// the group ends with the group's maximum.
+__device__ __forceinline__ int group_max_key(int key) {
+    key = max(key, __builtin_amdgcn_update_dpp(0, key, 0x128, 0xF, 0xF, false));
+    const auto s16 = __builtin_amdgcn_permlane16_swap(unsigned(key), unsigned(key), false, false);
+    key = max(int(s16[0]), int(s16[1]));
+    const auto s32 = __builtin_amdgcn_permlane32_swap(unsigned(key), unsigned(key), false, false);
+    return max(int(s32[0]), int(s32[1]));
 }
 
 // per-lane partial sums of the particles of one spin into the work area, [3][8][SP]: the up
@@ -682,20 +650,21 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
 #pragma unroll 1
         for (int ck = 0; ck < KN; ++ck) {
             const int k = 8 * BK + ck;
-            double best = -1.0;
-            int brow = 1 << 20;
+            // Partial pivoting with 32-bit keys: the high word of |S(i, k)| (monotonic in the
+            // magnitude) with its 6 lowest bits replaced by 63 - i.  The integer maximum is a
+            // largest candidate to 2^-14 relative, ties going to the lower row (any such
+            // pivot keeps the multipliers at most 1 + 2^-13 in magnitude).
+            int key = -1;
             if (c == ck) {
 #pragma unroll
                 for (int a = 0; a < TR; ++a) {
-                    const double v = fabs(Sm[a][BK]);
-                    if (!((used >> a) & 1u) && v > best) {
-                        best = v;
-                        brow = r + 8 * a;
-                    }
+                    const int hi = int(__double_as_longlong(Sm[a][BK]) >> 32) & 0x7fffffff;
+                    const int ka = (hi & ~63) | (63 - (r + 8 * a));
+                    key = ((used >> a) & 1u) ? key : max(key, ka);
                 }
             }
-            pivot_argmax(best, brow, tid);
-            const int p = __builtin_amdgcn_readlane(brow, ck);
+            key = group_max_key(key);
+            const int p = 63 - (__builtin_amdgcn_readlane(key, ck) & 63);
             const int ap = p >> 3;
             // the pivot from the lane that holds it (tile row ap of a lane of column group
             // ck) and its reciprocal, formed while the pivot row and column go through LDS
@@ -763,18 +732,18 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
     };
     static_for<0, TR>(pivot_block);
 
-    // permutation parity: n - #cycles
-    int ismin = 0;
+    // permutation parity: the parity of its inversions (i < j with perm[i] > perm[j]); lane
+    // i counts those of row i, from reads that do not depend on each other (a walk along
+    // the cycles would chain up to n dependent LDS reads)
+    int odd = 0;
     if (tid < NUP) {
-        ismin = 1;
-        int s = perm[tid];
-        for (int it = 0; it < NUP && s != tid; ++it) {
-            if (s < tid) { ismin = 0; break; }
-            s = perm[s];
-        }
+        const int pi = perm[tid];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < NUP; ++j) cnt += (j > tid && perm[j] < pi) ? 1 : 0;
+        odd = cnt & 1;
     }
-    const int ncyc = __popcll(__ballot(ismin));
-    if ((NUP - ncyc) & 1) sgn = -sgn;
+    if (__popcll(__ballot(odd)) & 1) sgn = -sgn;
     const double logdet = singular ? -1.0e30 : log(prod) + expo * 0.69314718055994530942;
     if (singular) sgn = 1;
 
