@@ -654,8 +654,12 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             // magnitude) with its 6 lowest bits replaced by 63 - i.  The integer maximum is a
             // largest candidate to 2^-14 relative, ties going to the lower row (any such
             // pivot keeps the multipliers at most 1 + 2^-13 in magnitude).
+            // The pivot column goes to LDS unscaled and first, as it depends on no pivot choice;
+            // its readers scale it by the pivot's reciprocal (the same products).
             int key = -1;
             if (c == ck) {
+#pragma unroll
+                for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][BK];
 #pragma unroll
                 for (int a = 0; a < TR; ++a) {
                     const int hi = int(__double_as_longlong(Sm[a][BK]) >> 32) & 0x7fffffff;
@@ -667,7 +671,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             const int p = 63 - (__builtin_amdgcn_readlane(key, ck) & 63);
             const int ap = p >> 3;
             // the pivot from the lane that holds it (tile row ap of a lane of column group
-            // ck) and its reciprocal, formed while the pivot row and column go through LDS
+            // ck) and its reciprocal, formed while the pivot row goes through LDS
             double pe = Sm[0][BK];
 #pragma unroll
             for (int a = 1; a < TR; ++a)
@@ -687,10 +691,6 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
 #pragma unroll
                 for (int b = 0; b < TR; ++b) prow[c + 8 * b] = rowp[b];
             }
-            if (c == ck) {
-#pragma unroll
-                for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][BK] * rp;  // multipliers
-            }
             if (tid == 0) {
                 perm[k] = p;
                 pinv[p] = k;
@@ -701,7 +701,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             prod *= fabs(piv);
             double f[TR], pv[TR];
 #pragma unroll
-            for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a];
+            for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a] * rp;  // multipliers
 #pragma unroll
             for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
             // elimination of every element, then the pivot column (lanes c == ck, tile
