@@ -35,63 +35,6 @@ __device__ __forceinline__ R rcp_r(R x) {
     else return __builtin_amdgcn_rcpf(x);
 }
 
-// sin and cos of 2 pi t for |t| <= 1/2 (t in turns), FP32: reduction to the nearest quarter
-// turn (|x| <= pi/4), the Taylor polynomials of sin to x^9 and of cos to x^10 (truncation
-// errors 1.8e-9 and 1.1e-10 at pi/4), then the rotation by that quarter turn
-__device__ __forceinline__ void sincos_turns(float t, float &s, float &c) {
-    const float q = rintf(4.0f * t);
-    const float x = fmaf(-0.25f, q, t) * 6.28318530717958648f;
-    const float x2 = x * x;
-    float sp = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
-    sp = fmaf(x2, sp, 8.33333333e-3f);
-    sp = fmaf(x2, sp, -1.66666667e-1f);
-    const float sn = fmaf(x * x2, sp, x);
-    float cp = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
-    cp = fmaf(x2, cp, -1.38888889e-3f);
-    cp = fmaf(x2, cp, 4.16666667e-2f);
-    cp = fmaf(x2, cp, -0.5f);
-    const float cs = fmaf(x2, cp, 1.0f);
-    // quarter turn iq (mod 4): odd iq swaps sin and cos; sin changes sign for iq = 2, 3 and
-    // cos for iq = 1, 2 (selects and sign flips, no branches)
-    const unsigned iq = unsigned(int(q));
-    const bool sw = iq & 1u;
-    const float s0 = sw ? cs : sn, c0 = sw ? sn : cs;
-    s = __uint_as_float(__float_as_uint(s0) ^ ((iq & 2u) << 30));
-    c = __uint_as_float(__float_as_uint(c0) ^ (((iq ^ (iq >> 1)) & 1u) << 31));
-}
-
-// The same in FP64: t - q/4 is exact (Sterbenz), x = 2 pi (t - q/4) in [-pi/4, pi/4] carries
-// one rounding; the Taylor polynomials of sin to x^17 and of cos to x^18 (coefficients
-// +-1/n!, truncation errors below 1e-19) in Horner form in x^2; the quarter turn as above
-__device__ __forceinline__ void sincos_turns(double t, double &s, double &c) {
-    const double q = rint(4.0 * t);
-    const double x = fma(-0.25, q, t) * 6.283185307179586477;
-    const double x2 = x * x;
-    double sp = fma(x2, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);  // 1/17!, -1/15!
-    sp = fma(x2, sp, 1.0 / 6227020800.0);
-    sp = fma(x2, sp, -1.0 / 39916800.0);
-    sp = fma(x2, sp, 1.0 / 362880.0);
-    sp = fma(x2, sp, -1.0 / 5040.0);
-    sp = fma(x2, sp, 1.0 / 120.0);
-    sp = fma(x2, sp, -1.0 / 6.0);
-    const double sn = fma(x * x2, sp, x);
-    double cp = fma(x2, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);  // -1/18!, 1/16!
-    cp = fma(x2, cp, -1.0 / 87178291200.0);
-    cp = fma(x2, cp, 1.0 / 479001600.0);
-    cp = fma(x2, cp, -1.0 / 3628800.0);
-    cp = fma(x2, cp, 1.0 / 40320.0);
-    cp = fma(x2, cp, -1.0 / 720.0);
-    cp = fma(x2, cp, 1.0 / 24.0);
-    cp = fma(x2, cp, -0.5);
-    const double cs = fma(x2, cp, 1.0);
-    const unsigned iq = unsigned(int(q));
-    const bool sw = iq & 1u;
-    const double s0 = sw ? cs : sn, c0 = sw ? sn : cs;
-    s = __longlong_as_double(__double_as_longlong(s0) ^ ((long long)((iq >> 1) & 1u) << 63));
-    c = __longlong_as_double(__double_as_longlong(c0) ^
-                             ((long long)((iq ^ (iq >> 1)) & 1u) << 63));
-}
-
 // natural log of a positive normal FP64 number: x = m 2^e with m in [1/sqrt 2, sqrt 2)
 // (frexp), ln m = 2 atanh(z) = 2 (z + z^3/3 + ... + z^19/19) with z = (m - 1)/(m + 1),
 // |z| <= 0.1716 (truncation below 1e-18), and e ln 2 with ln 2 split in a high part exact in

@@ -58,6 +58,63 @@ __device__ __forceinline__ double rcp_nr(double x) {
     return fma(y, e, y);
 }
 
+// sin and cos of 2 pi t for |t| <= 1/2 (t in turns), FP32: reduction to the nearest quarter
+// turn (|x| <= pi/4), the Taylor polynomials of sin to x^9 and of cos to x^10 (truncation
+// errors 1.8e-9 and 1.1e-10 at pi/4), then the rotation by that quarter turn
+__device__ __forceinline__ void sincos_turns(float t, float &s, float &c) {
+    const float q = rintf(4.0f * t);
+    const float x = fmaf(-0.25f, q, t) * 6.28318530717958648f;
+    const float x2 = x * x;
+    float sp = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+    sp = fmaf(x2, sp, 8.33333333e-3f);
+    sp = fmaf(x2, sp, -1.66666667e-1f);
+    const float sn = fmaf(x * x2, sp, x);
+    float cp = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);
+    cp = fmaf(x2, cp, -1.38888889e-3f);
+    cp = fmaf(x2, cp, 4.16666667e-2f);
+    cp = fmaf(x2, cp, -0.5f);
+    const float cs = fmaf(x2, cp, 1.0f);
+    // quarter turn iq (mod 4): odd iq swaps sin and cos; sin changes sign for iq = 2, 3 and
+    // cos for iq = 1, 2 (selects and sign flips, no branches)
+    const unsigned iq = unsigned(int(q));
+    const bool sw = iq & 1u;
+    const float s0 = sw ? cs : sn, c0 = sw ? sn : cs;
+    s = __uint_as_float(__float_as_uint(s0) ^ ((iq & 2u) << 30));
+    c = __uint_as_float(__float_as_uint(c0) ^ (((iq ^ (iq >> 1)) & 1u) << 31));
+}
+
+// The same in FP64: t - q/4 is exact (Sterbenz), x = 2 pi (t - q/4) in [-pi/4, pi/4] carries
+// one rounding; the Taylor polynomials of sin to x^17 and of cos to x^18 (coefficients
+// +-1/n!, truncation errors below 1e-19) in Horner form in x^2; the quarter turn as above
+__device__ __forceinline__ void sincos_turns(double t, double &s, double &c) {
+    const double q = rint(4.0 * t);
+    const double x = fma(-0.25, q, t) * 6.283185307179586477;
+    const double x2 = x * x;
+    double sp = fma(x2, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);  // 1/17!, -1/15!
+    sp = fma(x2, sp, 1.0 / 6227020800.0);
+    sp = fma(x2, sp, -1.0 / 39916800.0);
+    sp = fma(x2, sp, 1.0 / 362880.0);
+    sp = fma(x2, sp, -1.0 / 5040.0);
+    sp = fma(x2, sp, 1.0 / 120.0);
+    sp = fma(x2, sp, -1.0 / 6.0);
+    const double sn = fma(x * x2, sp, x);
+    double cp = fma(x2, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);  // -1/18!, 1/16!
+    cp = fma(x2, cp, -1.0 / 87178291200.0);
+    cp = fma(x2, cp, 1.0 / 479001600.0);
+    cp = fma(x2, cp, -1.0 / 3628800.0);
+    cp = fma(x2, cp, 1.0 / 40320.0);
+    cp = fma(x2, cp, -1.0 / 720.0);
+    cp = fma(x2, cp, 1.0 / 24.0);
+    cp = fma(x2, cp, -0.5);
+    const double cs = fma(x2, cp, 1.0);
+    const unsigned iq = unsigned(int(q));
+    const bool sw = iq & 1u;
+    const double s0 = sw ? cs : sn, c0 = sw ? sn : cs;
+    s = __longlong_as_double(__double_as_longlong(s0) ^ ((long long)((iq >> 1) & 1u) << 63));
+    c = __longlong_as_double(__double_as_longlong(c0) ^
+                             ((long long)((iq ^ (iq >> 1)) & 1u) << 63));
+}
+
 __device__ __forceinline__ double wrapd(double x, const DevSys &S) {
     return x - S.el * round(x * S.eli);  // Fortran nint: half away from zero
 }
@@ -300,7 +357,8 @@ __device__ __forceinline__ void axis_powers(const DevSys &S, const double *pos, 
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double s1, c1;
-        sincos(pos[d] * S.twopi_l, &s1, &c1);
+        const double t = pos[d] * S.eli;  // turns; the positions are wrapped into the box
+        sincos_turns(t - rint(t), s1, c1);
         pc[d][0] = 1.0; ps[d][0] = 0.0;
         pc[d][1] = c1; ps[d][1] = s1;
 #pragma unroll
@@ -341,9 +399,9 @@ __device__ __forceinline__ void phases_all(const DevSys &S, const double *xs, co
             const int e = t / nq, q = t - e * nq, p = 64 + e;
             const int4 m = qm[q];
             const double *pos = xs + 3 * p;
-            const double th = S.twopi_l * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
+            const double tq = S.eli * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
             double sn, cs;
-            sincos(th, &sn, &cs);
+            sincos_turns(tq - rint(tq), sn, cs);
             ph[q * NP + p] = make_double2(cs, sn);
         }
     }
@@ -375,9 +433,9 @@ __device__ __forceinline__ void phases_range(const DevSys &S, const double *xs, 
             const int e = t / nqc, qq = t - e * nqc, p = 64 + e;
             const int4 m = qm[q0 + qq];
             const double *pos = xs + 3 * p;
-            const double th = S.twopi_l * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
+            const double tq = S.eli * (m.x * pos[0] + m.y * pos[1] + m.z * pos[2]);
             double sn, cs;
-            sincos(th, &sn, &cs);
+            sincos_turns(tq - rint(tq), sn, cs);
             ph[qq * NP + p] = make_double2(cs, sn);
         }
     }
