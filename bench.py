@@ -558,7 +558,7 @@ def cpu_port_baseline():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=1000)  # the headline configuration's length
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed",
                     choices=["mixed", "fp64", "allparticle", "batch", "observables"])
@@ -624,7 +624,7 @@ def main():
         print(f"bench: precision: {extra['precision']}", file=sys.stderr, flush=True)
         if not a.no_energy:
             extra["dmc_energy"] = energy_study()
-        br = batch_run(min(a.steps, 30), 3)
+        br = batch_run(2 * a.steps, 3)  # 2000 steps by default, the batch configuration's
         print(f"bench: batch: {br}", file=sys.stderr, flush=True)
         extra["batch"] = br
     sm = a.workload != "allparticle"
