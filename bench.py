@@ -220,8 +220,9 @@ def _series_mean(x):
     / n) with the integrated autocorrelation time tau_int summed up to the first lag
     M >= 5 tau_int (the automatic window of Madras and Sokal, J. Stat. Phys. 50 (1988) 109).
     Returns (mean, error, tau_int in steps, steps left out)."""
-    med = float(np.median(x))
-    mad = 1.4826 * float(np.median(np.abs(x - med)))
+    fin = x[np.isfinite(x)]  # a step whose estimator weights sum to zero gives 0/0
+    med = float(np.median(fin))
+    mad = 1.4826 * float(np.median(np.abs(fin - med)))
     keep = np.isfinite(x) & (np.abs(x - med) <= 10.0 * max(mad, 1e-12))
     y = x[keep]
     n = len(y)
