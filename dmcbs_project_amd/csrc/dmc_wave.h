@@ -374,8 +374,15 @@ __device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const
                                           double2 *out, int stride) {
     double pc[3][5], ps[3][5];
     axis_powers(S, pos, pc, ps);
+    // the q vectors through scalar loads of the system's read-only table (wave-uniform, so
+    // the powers are picked by scalar branches without a move from vector registers)
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(4))) const v4i cv4i;
+    const cv4i *qk = (const cv4i *)(S.qm);
+    (void)qm;
     for (int q = 0; q < S.nq; ++q) {
-        const int4 m = qm[q];
+        const v4i mv = qk[q];
+        const int4 m = make_int4(mv.x, mv.y, mv.z, mv.w);
         double cx, sx, cy, sy, cz, sz;
         cpow(pc[0], ps[0], m.x, cx, sx);
         cpow(pc[1], ps[1], m.y, cy, sy);
@@ -656,6 +663,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
         }
     // the plane-wave terms of q vectors q0 <= q < q1, whose phases are at ph[(q - q0) * NP]
     auto add_waves = [&](int q0, int q1) __attribute__((always_inline)) {
+#pragma unroll 2
         for (int q = q0; q < q1; ++q) {
             const double w = qc[5 * q];
             const double2 *pq = ph + (q - q0) * NP;
