@@ -336,18 +336,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // exp(i m theta) from the precomputed powers, |m| <= 4.  m is wave-uniform (the same q
-// for every lane): scalar branches pick the power, no per-lane selects.
+// for every lane): the power is picked by a register index, no per-lane selects.
 __device__ __forceinline__ void cpow(const double (&pc)[5], const double (&ps)[5], int m,
                                      double &c, double &s) {
     m = __builtin_amdgcn_readfirstlane(m);
-    const int am = m < 0 ? -m : m;
-    switch (am) {
-        case 0: c = 1.0; s = 0.0; break;
-        case 1: c = pc[1]; s = ps[1]; break;
-        case 2: c = pc[2]; s = ps[2]; break;
-        case 3: c = pc[3]; s = ps[3]; break;
-        default: c = pc[4]; s = ps[4]; break;
-    }
+    int am = m < 0 ? -m : m;
+    am = am > 4 ? 4 : am;
+    c = pc[am];  // a uniform index into registers (relative moves, no branches)
+    s = ps[am];
     s = m < 0 ? -s : s;
 }
 
@@ -375,7 +371,7 @@ __device__ __forceinline__ void phases_of(const DevSys &S, const int4 *qm, const
     double pc[3][5], ps[3][5];
     axis_powers(S, pos, pc, ps);
     // the q vectors through scalar loads of the system's read-only table (wave-uniform, so
-    // the powers are picked by scalar branches without a move from vector registers)
+    // the powers are picked by a register index without a move from vector registers)
     typedef int v4i __attribute__((ext_vector_type(4)));
     typedef __attribute__((address_space(4))) const v4i cv4i;
     const cv4i *qk = (const cv4i *)(S.qm);
@@ -875,15 +871,24 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, const TB &tb, do
             // r and 1/r from one reciprocal square root, r floored at 1e-14 (coshpot.f90:167;
             // the floor also keeps coincident particles finite in the beta lookup)
             const double irj = rsqrt_nr(fmax(r2, 1.0e-28)), rj = r2 * irj, rr = rj;
-            if (rad) rad->v[a][b] = S.alpha0 + beta_val(tb, S, rr);
+            // the beta and potential nodes of the pair are loaded together, before either
+            // sum (intervals clamped to the table; out-of-range terms are dropped below)
+            double wb0, wb1, wb2, wb3, w0, w1, w2, w3;
+            const int ib = lagrange(S.scaleb * rr, S.ntab, wb0, wb1, wb2, wb3);
+            const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+            const double b0 = tb.bt1(ib - 1), b1 = tb.bt1(ib), b2 = tb.bt1(ib + 1),
+                         b3 = tb.bt1(ib + 2);
+            const double v0 = tb.vt1(idx - 1), v1 = tb.vt1(idx), v2 = tb.vt1(idx + 1),
+                         v3 = tb.vt1(idx + 2);
+            if (rad)
+                rad->v[a][b] = S.alpha0 + ((S.nobeta || rr > S.rangeb)
+                                               ? 0.0
+                                               : wb0 * b0 + wb1 * b1 + wb2 * b2 + wb3 * b3);
             if (rj < S.range) {
-                double w0, w1, w2, w3;
-                const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
                 if (idx >= S.jfar) {
                     // past the healing distance u, u' and u'' are zero on the whole stencil:
                     // the potential alone, the same arithmetic as the record's first field
-                    V += w0 * tb.vt1(idx - 1) + w1 * tb.vt1(idx) + w2 * tb.vt1(idx + 1) +
-                         w3 * tb.vt1(idx + 2);
+                    V += w0 * v0 + w1 * v1 + w2 * v2 + w3 * v3;
                 } else {
                     const double4 jv = lookup4j(tb, idx, w0, w1, w2, w3);
                     V += jv.x;
