@@ -465,7 +465,7 @@ struct Layout {
     static constexpr int SP = (NUP + 7) / 16 * 16 + 8;
     // q vectors per chunk of phases in the compact layout
     static constexpr int kQChunk = 8;
-    int xs, qc, qm, dp, xn, prow, mcol, ints, work, sinv, est, scal, total;
+    int xs, qc, qm, dp, xn, mcol, ints, work, sinv, est, scal, total;
     int qch;  // q vectors whose phases the work area holds at once
     bool lean;
     __host__ __device__ explicit Layout(int nq, bool lean_ = false, bool compact = false)
@@ -481,8 +481,7 @@ struct Layout {
         if (!lean) {
             dp = qm + 2 * nq;
             xn = dp + 3 * NP;                 // one extra position (4) and its phases (2 nq)
-            prow = xn + 4 + 2 * nq;
-            mcol = prow + R8;
+            mcol = xn + 4 + 2 * nq;
             ints = mcol + R8;                 // perm[64], pinv[64]
             work = ints + 64;
             sinv = work + phs;
@@ -492,8 +491,7 @@ struct Layout {
             work = qm + 2 * nq;
             sinv = work;
             ints = sinv + inv;                // live until the inverse is scattered
-            prow = ints + 64;
-            mcol = prow + R8;
+            mcol = ints + 64;
             dp = work;                        // written once the partial sums are read
             est = dp + 3 * NP;
             xn = -1;                          // no extra position
@@ -694,11 +692,14 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
     }
 
     // Phase C: Gauss-Jordan inversion with partial pivoting, rows never moved
-    double *prow = sm + L.prow, *mcol = sm + L.mcol;
-    unsigned used = 0;
+    // The pivot row goes through the area of the inverse, which is written only once the
+    // elimination ends.
+    double *prow = sm + L.sinv, *mcol = sm + L.mcol;
+    // low bits of the pivot keys: 63 - i for a row i not yet a pivot row; the sign bit is set
+    // once it is one (and for the padding rows), which makes every key of that row negative
+    int klo[TR];
 #pragma unroll
-    for (int a = 0; a < TR; ++a)
-        if (!rv[a]) used |= 1u << a;
+    for (int a = 0; a < TR; ++a) klo[a] = rv[a] ? 63 - (r + 8 * a) : int(0x80000000u);
     double prod = 1.0;
     int expo = 0, sgn = 1;
     bool singular = false;
@@ -724,9 +725,8 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
                 for (int a = 0; a < TR; ++a) mcol[r + 8 * a] = Sm[a][BK];
 #pragma unroll
                 for (int a = 0; a < TR; ++a) {
-                    const int hi = int(__double_as_longlong(Sm[a][BK]) >> 32) & 0x7fffffff;
-                    const int ka = (hi & ~63) | (63 - (r + 8 * a));
-                    key = ((used >> a) & 1u) ? key : max(key, ka);
+                    const unsigned hi = unsigned(__double_as_longlong(Sm[a][BK]) >> 32);
+                    key = max(key, int((hi & 0x7fffffc0u) | unsigned(klo[a])));
                 }
             }
             key = group_max_key(key);
@@ -739,24 +739,21 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             for (int a = 1; a < TR; ++a)
                 if (a == ap) pe = Sm[a][BK];
             const double piv = readlane_d(pe, 8 * (p & 7) + ck);
-            const double rp = rcp_nr(piv);
-            if (r == (p & 7)) {
-                double rowp[TR];
+            double rp = rcp_nr(piv);
+            pin(rp);  // issued ahead of the pivot-row stores, which cover its latency
+            // the pivot row from tile row ap, in a scalar branch of its own (the empty asm
+            // statement, different in each branch, keeps them from merging into selects)
+            auto put_row = [&](auto ac) __attribute__((always_inline)) {
+                constexpr int A = decltype(ac)::value;
+                if (A == ap) {
+                    if (r == (p & 7)) {
 #pragma unroll
-                for (int b = 0; b < TR; ++b) rowp[b] = Sm[0][b];
-#pragma unroll
-                for (int a = 1; a < TR; ++a)
-                    if (a == ap) {
-#pragma unroll
-                        for (int b = 0; b < TR; ++b) rowp[b] = Sm[a][b];
+                        for (int b = 0; b < TR; ++b) prow[c + 8 * b] = Sm[A][b];
                     }
-#pragma unroll
-                for (int b = 0; b < TR; ++b) prow[c + 8 * b] = rowp[b];
-            }
-            if (tid == 0) {
-                perm[k] = p;
-                pinv[p] = k;
-            }
+                    asm volatile("; pivot row %0" ::"n"(A) : "memory");
+                }
+            };
+            static_for<0, TR>(put_row);
             wave_sync();
             singular |= (piv == 0.0);
             sgn = piv < 0.0 ? -sgn : sgn;
@@ -766,6 +763,11 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a] * rp;  // multipliers
 #pragma unroll
             for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
+            // the permutation behind the reads of the pivot row and column in the LDS queue
+            if (tid == 0) {
+                perm[k] = p;
+                pinv[p] = k;
+            }
             // elimination of every element, then the pivot column (lanes c == ck, tile
             // column BK) and the pivot row (lanes r == p & 7, tile row ap) overwritten
 #pragma unroll
@@ -783,7 +785,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
                         const double prw = (b == BK && colk_lane) ? rp : pv[b] * rp;
                         Sm[a][b] = prow_lane ? prw : Sm[a][b];
                     }
-                    if (prow_lane) used |= 1u << a;
+                    if (prow_lane) klo[a] |= int(0x80000000u);
                 }
             wave_sync();
         }
