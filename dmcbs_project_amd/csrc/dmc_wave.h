@@ -703,6 +703,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
     double prod = 1.0;
     int expo = 0, sgn = 1;
     bool singular = false;
+    int permk = 0;  // lane k: the pivot row of step k
     // The pivots in blocks of eight, one per tile column BK (a compile-time index, so the
     // pivot column is read and overwritten in place), one pivot per iteration of a rolled
     // loop within a block (a compact body whose registers are those of one step).  Tile row
@@ -755,19 +756,19 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
             };
             static_for<0, TR>(put_row);
             wave_sync();
-            singular |= (piv == 0.0);
-            sgn = piv < 0.0 ? -sgn : sgn;
+            {
+                // sign and zero test on the pivot's bits (wave-uniform, scalar operations)
+                const long long pb = __double_as_longlong(piv);
+                singular |= (pb & 0x7fffffffffffffffll) == 0;
+                sgn = pb < 0 ? -sgn : sgn;
+            }
             prod *= fabs(piv);
             double f[TR], pv[TR];
 #pragma unroll
             for (int a = 0; a < TR; ++a) f[a] = mcol[r + 8 * a] * rp;  // multipliers
 #pragma unroll
             for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
-            // the permutation behind the reads of the pivot row and column in the LDS queue
-            if (tid == 0) {
-                perm[k] = p;
-                pinv[p] = k;
-            }
+            permk = tid == k ? p : permk;
             // elimination of every element, then the pivot column (lanes c == ck, tile
             // column BK) and the pivot row (lanes r == p & 7, tile row ap) overwritten
 #pragma unroll
@@ -795,13 +796,18 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
         expo += e;
     };
     static_for<0, TR>(pivot_block);
+    if (tid < NUP) {
+        perm[tid] = permk;
+        pinv[permk] = tid;
+    }
+    wave_sync();
 
     // permutation parity: the parity of its inversions (i < j with perm[i] > perm[j]); lane
     // i counts those of row i, from reads that do not depend on each other (a walk along
     // the cycles would chain up to n dependent LDS reads)
     int odd = 0;
     if (tid < NUP) {
-        const int pi = perm[tid];
+        const int pi = permk;
         int cnt = 0;
 #pragma unroll
         for (int j = 0; j < NUP; ++j) cnt += (j > tid && perm[j] < pi) ? 1 : 0;
