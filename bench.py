@@ -239,31 +239,69 @@ def _series_mean(x):
     return float(y.mean()), se, tau, int((~keep).sum())
 
 
-def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000):
+def _replica_stats(reps):
+    """one energy from R independent replicas, each a list (mean, error, tau_int, steps left
+    out) from _series_mean: the mean of the replica means (equal populations), and two
+    errors, the spread of the replica means, std / sqrt(R), which needs no model of the
+    correlation in time, and the replicas' autocorrelation errors combined, sqrt(sum se^2) / R.
+    The quoted error is the larger of the two."""
+    m = np.array([r[0] for r in reps], float)
+    se = np.array([r[1] for r in reps], float)
+    nrep = len(reps)
+    err_rep = float(m.std(ddof=1) / np.sqrt(nrep)) if nrep > 1 else float("nan")
+    err_ac = float(np.sqrt(np.sum(se ** 2)) / nrep)
+    err = max(err_rep, err_ac) if np.isfinite(err_rep) else err_ac
+    return dict(e=float(m.mean()), err=err, err_replicas=err_rep, err_autocorr=err_ac,
+                replica_means=[float(v) for v in m],
+                tau_int=float(np.mean([r[2] for r in reps])),
+                outlier_steps=int(sum(r[3] for r in reps)))
+
+
+def _quadratic_extrapolation(taus, es, errs):
+    """E(0) of the weighted least-squares fit E(tau) = E0 + a tau + b tau^2, its error from
+    the fit's covariance, and chi^2 with its degrees of freedom"""
+    t = np.asarray(taus, float)
+    y = np.asarray(es, float)
+    w = 1.0 / np.asarray(errs, float) ** 2
+    A = np.stack([np.ones_like(t), t, t * t], axis=1)
+    cov = np.linalg.inv(A.T @ (A * w[:, None]))
+    coef = cov @ (A.T @ (w * y))
+    chi2 = float(np.sum(w * (A @ coef - y) ** 2))
+    return dict(e=float(coef[0]), err=float(np.sqrt(cov[0, 0])), chi2=chi2, dof=len(t) - 3)
+
+
+def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
     """DMC energies of the headline system from an equilibrated population.
 
     One population (32768 walkers) is equilibrated by nequil mixed-precision steps at tau
     from the reference's initial walkers (copies of one configuration; their energy settles
-    after about 4.5 units of imaginary time).  From it:
+    after about 4.5 units of imaginary time), then split into nrep groups, each run as an
+    independent population (its own trial energy and population control) of 32768 / nrep
+    walkers for every propagator below, so that each energy has nrep independent replicas
+    (_replica_stats):
     * the mixed and the FP64 single-particle sweeps at tau (same walkers and random streams;
-      the FP64 run is nfp64 steps, compared with the mixed run's first nfp64 steps), and the
-      mixed sweep at tau/2 (nsm steps each, tau/2 after 1000 steps of re-equilibration);
+      the FP64 run is nfp64 steps, compared replica by replica with the mixed run's first
+      nfp64 steps), and the mixed sweep at tau/2 (nsm steps each, tau/2 after 1000 steps of
+      re-equilibration);
     * the reference's own algorithm, the all-particle step (the device replay that reproduces
-      the reference's trajectories), at tau, tau/2 and tau/4: nap f steps after 1000 f steps
-      (one unit of imaginary time) of re-equilibration (f = 1, 2, 4).  Its time-step error is large and not linear at these
-      steps, so E(0) comes from the quadratic through the three, (8 E(tau/4) - 6 E(tau/2) +
-      E(tau)) / 3.
+      the reference's trajectories), at tau, tau/2, tau/4 and tau/8: nap f steps after 1000 f
+      steps (one unit of imaginary time) of re-equilibration (f = 1, 2, 4, 8).  Its time-step
+      error is large and not linear at these steps: E(0) comes from the quadratic through the
+      three smallest steps, (8 E(tau/8) - 6 E(tau/4) + E(tau/2)) / 3, and, as a check of that
+      model, from the weighted least-squares quadratic through all four (with its chi^2).
     Every propagator cuts the drift at 2 sigma (the reference's dcut, step.f90:32-37): with
     the deck's dcut = -1 the all-particle step from an equilibrated population lets walkers
     next to a node overshoot into regions of very negative local energy whose uncapped
-    weights swamp the estimator (E = -132 +- 132 in one such run).  Energies and errors per
-    run come from the per-step energies (_series_mean)."""
+    weights swamp the estimator (E = -132 +- 132 in one such run).  Each replica's energy
+    comes from its per-step energies (_series_mean)."""
     e = make_engine("mixed")
     for _ in range(nequil):
         e.step()
     w = e.walkers()
-    walkers = (w["x"], w["irn"])
     e.close()
+    per = len(w["irn"]) // nrep
+    groups = [(w["x"][r * per:(r + 1) * per], w["irn"][r * per:(r + 1) * per])
+              for r in range(nrep)]
     series, runs = {}, {}
     for label, mode, dt, neq, nmeas in (("mixed", "mixed", DT, 0, nsm),
                                          ("fp64", "fp64", DT, 0, nfp64),
@@ -272,12 +310,16 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000):
                                          ("allparticle_half_tau", "allparticle", DT / 2, 2000,
                                           2 * nap),
                                          ("allparticle_quarter_tau", "allparticle", DT / 4, 4000,
-                                          4 * nap)):
-        e = make_engine(mode, dt=dt, walkers=walkers, dcut=DCUT_SM)
-        series[label] = _energy_series(e, neq, nmeas)
-        e.close()
-        m, se, tau, nout = _series_mean(series[label])
-        runs[label] = dict(e=m, err=se, steps=nmeas, tau_int=tau, outlier_steps=nout)
+                                          4 * nap),
+                                         ("allparticle_eighth_tau", "allparticle", DT / 8, 8000,
+                                          8 * nap)):
+        series[label] = []
+        for g in groups:
+            e = make_engine(mode, nwalk=per, dt=dt, walkers=g, dcut=DCUT_SM)
+            series[label].append(_energy_series(e, neq, nmeas))
+            e.close()
+        runs[label] = dict(_replica_stats([_series_mean(x) for x in series[label]]),
+                           steps=nmeas)
         print(f"bench: energy {label}: {runs[label]}", file=sys.stderr, flush=True)
 
     def diff(a, b):
@@ -288,23 +330,41 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000):
     def ev(label):
         return runs[label]["e"], runs[label]["err"]
 
-    m1 = _series_mean(series["mixed"][:nfp64])
-    e1, e2, e4 = ev("allparticle"), ev("allparticle_half_tau"), ev("allparticle_quarter_tau")
-    ap0 = ((8 * e4[0] - 6 * e2[0] + e1[0]) / 3,
-           float(np.sqrt((8 * e4[1]) ** 2 + (6 * e2[1]) ** 2 + e1[1] ** 2) / 3))
-    ap0_lin = (2 * e4[0] - e2[0], float(np.hypot(2 * e4[1], e2[1])))
+    # mixed against FP64 replica by replica: the two start from the same walkers and streams
+    pairs = [_series_mean(a[:nfp64])[0] - _series_mean(b)[0]
+             for a, b in zip(series["mixed"], series["fp64"])]
+    dpair = float(np.mean(pairs))
+    spair = float(np.std(pairs, ddof=1) / np.sqrt(nrep))
+    e2, e4, e8 = (ev("allparticle_half_tau"), ev("allparticle_quarter_tau"),
+                  ev("allparticle_eighth_tau"))
+    ap0 = ((8 * e8[0] - 6 * e4[0] + e2[0]) / 3,
+           float(np.sqrt((8 * e8[1]) ** 2 + (6 * e4[1]) ** 2 + e2[1] ** 2) / 3))
+    ap0_lin = (2 * e8[0] - e4[0], float(np.hypot(2 * e8[1], e4[1])))
+    ap_labels = ("allparticle", "allparticle_half_tau", "allparticle_quarter_tau",
+                 "allparticle_eighth_tau")
+    lsq = _quadratic_extrapolation([DT, DT / 2, DT / 4, DT / 8],
+                                   [ev(k)[0] for k in ap_labels], [ev(k)[1] for k in ap_labels])
     sm = ev("mixed_half_tau")
     return dict(unit="E_L / (N E_FG)", tau=DT,
-                start=f"{nequil} mixed steps from the reference's initial walkers",
-                error="per-step energies, integrated autocorrelation time (automatic "
-                      "window); steps beyond 10 robust sigma left out",
+                start=f"{nequil} mixed steps from the reference's initial walkers, then "
+                      f"{nrep} independent populations of {per} walkers",
+                error=f"{nrep} replicas per energy: the larger of the spread of the replica "
+                      "means and their autocorrelation errors (per-step energies, integrated "
+                      "autocorrelation time, automatic window; steps beyond 10 robust sigma "
+                      "left out)",
                 runs=runs,
-                mixed_vs_fp64=dict(diff(m1, ev("fp64")), steps=nfp64),
+                mixed_vs_fp64=dict(diff=dpair, sigma=spair,
+                                   diff_over_sigma=dpair / spair if spair > 0 else None,
+                                   steps=nfp64, paired="replica by replica"),
                 singleparticle_tau_vs_half_tau=diff(ev("mixed"), sm),
-                allparticle_extrapolated=dict(e=ap0[0], err=ap0[1], fit="quadratic in tau"),
+                allparticle_extrapolated=dict(e=ap0[0], err=ap0[1],
+                                              fit="quadratic through tau/2, tau/4, tau/8"),
+                allparticle_extrapolated_lsq=dict(lsq, fit="weighted least-squares quadratic "
+                                                           "through tau .. tau/8"),
                 allparticle_extrapolated_linear=dict(e=ap0_lin[0], err=ap0_lin[1],
-                                                     fit="tau/2, tau/4"),
-                singleparticle_vs_allparticle_extrapolated=diff(sm, ap0))
+                                                     fit="tau/4, tau/8"),
+                singleparticle_vs_allparticle_extrapolated=diff(sm, ap0),
+                singleparticle_vs_allparticle_extrapolated_lsq=diff(sm, (lsq["e"], lsq["err"])))
 
 
 BATCH_SYSTEMS = [(14, 3, 8192), (38, 4, 16384), (66, 5, 24576)]
