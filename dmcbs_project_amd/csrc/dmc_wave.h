@@ -879,19 +879,48 @@ __device__ __forceinline__ JSums jastrow_terms(const DevSys &S, const TB &tb, do
             // r and 1/r from one reciprocal square root, r floored at 1e-14 (coshpot.f90:167;
             // the floor also keeps coincident particles finite in the beta lookup)
             const double irj = rsqrt_nr(fmax(r2, 1.0e-28)), rj = r2 * irj, rr = rj;
-            // the beta and potential nodes of the pair are loaded together, before either
-            // sum (intervals clamped to the table; out-of-range terms are dropped below)
-            double wb0, wb1, wb2, wb3, w0, w1, w2, w3;
-            const int ib = lagrange(S.scaleb * rr, S.ntab, wb0, wb1, wb2, wb3);
-            const int idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
-            const double b0 = tb.bt1(ib - 1), b1 = tb.bt1(ib), b2 = tb.bt1(ib + 1),
-                         b3 = tb.bt1(ib + 2);
-            const double v0 = tb.vt1(idx - 1), v1 = tb.vt1(idx), v2 = tb.vt1(idx + 1),
-                         v3 = tb.vt1(idx + 2);
-            if (rad)
-                rad->v[a][b] = S.alpha0 + ((S.nobeta || rr > S.rangeb)
-                                               ? 0.0
-                                               : wb0 * b0 + wb1 * b1 + wb2 * b2 + wb3 * b3);
+            double w0, w1, w2, w3, v0, v1, v2, v3;
+            int idx;
+            // the beta sum is pinned before alpha_1 is added, so that its roundings do not
+            // depend on how the code around it is arranged (both table views below give the
+            // same bits)
+            if constexpr (!TB::kCompact) {
+                // tables in global memory: the beta and potential nodes of the pair are loaded
+                // together, before either sum (intervals clamped to the table; out-of-range
+                // terms are dropped below), so that one L1 round trip covers both
+                double wb0, wb1, wb2, wb3;
+                const int ib = lagrange(S.scaleb * rr, S.ntab, wb0, wb1, wb2, wb3);
+                idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+                const double b0 = tb.bt1(ib - 1), b1 = tb.bt1(ib), b2 = tb.bt1(ib + 1),
+                             b3 = tb.bt1(ib + 2);
+                v0 = tb.vt1(idx - 1);
+                v1 = tb.vt1(idx);
+                v2 = tb.vt1(idx + 1);
+                v3 = tb.vt1(idx + 2);
+                if (rad) {
+                    double bs = (S.nobeta || rr > S.rangeb)
+                                    ? 0.0
+                                    : wb0 * b0 + wb1 * b1 + wb2 * b2 + wb3 * b3;
+                    pin(bs);
+                    rad->v[a][b] = S.alpha0 + bs;
+                }
+            } else {
+                // tables in LDS (short latency): each lookup where it is used, which keeps
+                // fewer values live across the pair
+                if (rad) {
+                    double bs = beta_val(tb, S, rr);
+                    pin(bs);
+                    rad->v[a][b] = S.alpha0 + bs;
+                }
+                idx = lagrange(S.scale * rj, S.ntab, w0, w1, w2, w3);
+                v0 = v1 = v2 = v3 = 0.0;
+                if (rj < S.range && idx >= S.jfar) {
+                    v0 = tb.vt1(idx - 1);
+                    v1 = tb.vt1(idx);
+                    v2 = tb.vt1(idx + 1);
+                    v3 = tb.vt1(idx + 2);
+                }
+            }
             if (rj < S.range) {
                 if (idx >= S.jfar) {
                     // past the healing distance u, u' and u'' are zero on the whole stencil:
