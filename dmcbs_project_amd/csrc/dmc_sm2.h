@@ -73,6 +73,10 @@ __device__ __forceinline__ uint64_t half_lane(uint64_t v, int j, bool upper) {
     const uint64_t a = lane_value(v, j), b = lane_value(v, j + 32);
     return upper ? b : a;
 }
+__device__ __forceinline__ double half_lane(double v, int j, bool upper) {
+    const double a = lane_value(v, j), b = lane_value(v, j + 32);
+    return upper ? b : a;
+}
 
 // v of the other lane of an even/odd lane pair (DPP quad_perm 1,0,3,2)
 __device__ __forceinline__ float pair_swap(float v) {
@@ -99,11 +103,12 @@ struct PairLayout {
     static_assert(PL + (NUP * TS + 1) / 2 <= WALKER, "transposition area");
     static_assert(PL % 2 == 0 && WB % 2 == 0 && WALKER % 2 == 0, "16-byte alignment");
     // shared: per q the weights w_q, -w_q q (FP32), the integer q vectors, the jump-ahead
-    // coefficients of draws 1..5
+    // coefficients of draws 1..NJ (the draws of a batch of kRngBatch moves)
+    static constexpr int NJ = 5 * 10;
     static constexpr int QK = 2 * WALKER;
     static constexpr int QM = QK + 2 * NQMAX;
     static constexpr int JMP = QM + 2 * NQMAX;
-    static constexpr int TOTAL = JMP + 10;
+    static constexpr int TOTAL = JMP + 2 * NJ;
 };
 
 template <int NUP>
@@ -245,9 +250,10 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
         qk[4 * q + 2] = float(-wq * tp * m.y);
         qk[4 * q + 3] = float(-wq * tp * m.z);
     }
-    if (tid < 5) {
+    // (the system's jump-ahead table holds the first 1024 draws)
+    if (tid < L::NJ) {
         jmp[tid] = S.ja1[tid];
-        jmp[5 + tid] = S.jc1[tid];
+        jmp[L::NJ + tid] = S.jc1[tid];
     }
     for (int t = k; t < M; t += 32) xs[t] = wrapd(xin[size_t(w) * M + t], S);
     wave_sync();
@@ -308,7 +314,15 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
     const float alpha0 = float(S.alpha0);
     uint64_t s = irn_in[w];
     int nacc = 0;
-    const int jl0 = k < 3 ? 2 * k : -1, jl1 = k < 2 ? 2 * k + 1 : -1;
+    // the draws of a batch of up to 10 moves, formed every 10th move: lane 3m + i of the
+    // half (m < 10) holds those of the batch's move m, lane 3m draws 5m + 1, 5m + 2 (a
+    // gaussian pair), lane 3m + 1 draws 5m + 3, 5m + 4, lane 3m + 2 draw 5m + 5 (the
+    // acceptance draw; the walker's state after the move)
+    constexpr int kRngBatch = 10;
+    static_assert(3 * kRngBatch <= 32 && 5 * kRngBatch == L::NJ, "draw batch");
+    const int bm = k / 3, bi = k - 3 * bm;
+    float bgx = 0.0f, bgy = 0.0f;
+    double bu = 0.0;
     float ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};  // mover with partner k (k < NB)
     float pc = 0, ps = 0;  // exp(i q.x_a) of the mover's current position (this lane's q)
     bool have_old = false;
@@ -418,23 +432,32 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
         g2 = half_total(g2);
         const float go[3] = {g0, g1, g2};
         // gaussian(3) then the acceptance draw: five ran1 draws per move by jump-ahead from
-        // the walker state in lanes 0-2 of the half, broadcast over the half
+        // the walker state, formed for a batch of moves at once and broadcast over the half
         float g3[3];
         double u4;
         {
-            const uint64_t ja0 = jl0 >= 0 ? jmp[jl0] : 0, jc0 = jl0 >= 0 ? jmp[5 + jl0] : 0;
-            const uint64_t ja1 = jl1 >= 0 ? jmp[jl1] : 0, jc1 = jl1 >= 0 ? jmp[5 + jl1] : 0;
-            const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
-            const double ua = double(sa | 1ull) * 0x1p-48, ubd = double(sb | 1ull) * 0x1p-48;
-            float sn, cs;
-            sincos_turns(float(ua - rint(ua)), sn, cs);
-            const float rad = sqrt_r<float>(-2.0f * log_r<float>(float(ubd)));
-            const float gx = sn * rad, gy = cs * rad;
-            g3[0] = half_lane(gx, 0, upper);
-            g3[1] = half_lane(gy, 0, upper);
-            g3[2] = half_lane(gy, 1, upper);
-            s = half_lane(sa, 2, upper);
-            u4 = double(s | 1ull) * 0x1p-48;
+            const int ml = (ip - m0) % kRngBatch;  // wave-uniform
+            if (ml == 0) {
+                const int cnt = m1 - ip < kRngBatch ? m1 - ip : kRngBatch;
+                const bool on = bm < cnt;
+                const bool on1 = on && bi < 2;
+                const int t0 = on ? 5 * bm + 2 * bi : 0, t1 = on1 ? t0 + 1 : 0;  // < NJ
+                const uint64_t ja0 = on ? jmp[t0] : 0, jc0 = on ? jmp[L::NJ + t0] : 0;
+                const uint64_t ja1 = on1 ? jmp[t1] : 0, jc1 = on1 ? jmp[L::NJ + t1] : 0;
+                const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
+                const double ua = double(sa | 1ull) * 0x1p-48, ubd = double(sb | 1ull) * 0x1p-48;
+                float sn, cs;
+                sincos_turns(float(ua - rint(ua)), sn, cs);
+                const float rad = sqrt_r<float>(-2.0f * log_r<float>(float(ubd)));
+                bgx = sn * rad;
+                bgy = cs * rad;
+                bu = ua;  // the acceptance draw in the lanes 3m + 2
+                s = half_lane(sa, 3 * cnt - 1, upper);  // the state after the batch
+            }
+            g3[0] = half_lane(bgx, 3 * ml, upper);
+            g3[1] = half_lane(bgy, 3 * ml, upper);
+            g3[2] = half_lane(bgy, 3 * ml + 1, upper);
+            u4 = half_lane(bu, 3 * ml + 2, upper);
         }
         float dro[3], dxm[3];
         double xn[3];
