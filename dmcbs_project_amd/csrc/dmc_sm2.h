@@ -93,13 +93,16 @@ struct PairLayout {
     // per walker: positions [NP][3] FP64; the partners' phases [q][ST] complex FP32, which
     // the transposition of the inverse [NUP][TS] FP32 overlays at the switch of spins
     // (reaching into the weights, which are dead there); the mover's weights w_b and the
-    // new pair values f_b (33 FP32 each, in 144 B); w_q exp(i q.x') (32 FP32)
+    // new pair values f_b (33 FP32 each, in 144 B); w_q exp(i q.x') (32 FP32); the draws of a
+    // batch of 10 moves, 32 B per move (the three gaussians as FP32, the acceptance draw as
+    // FP64)
     static constexpr int XS = 0;
     static constexpr int PL = XS + 3 * NP;
     static constexpr int WB = PL + NQMAX * ST;
     static constexpr int FB = WB + 18;
     static constexpr int UB = FB + 18;
-    static constexpr int WALKER = UB + 16;
+    static constexpr int RB = UB + 16;
+    static constexpr int WALKER = RB + 4 * 10;
     static_assert(PL + (NUP * TS + 1) / 2 <= WALKER, "transposition area");
     static_assert(PL % 2 == 0 && WB % 2 == 0 && WALKER % 2 == 0, "16-byte alignment");
     // shared: per q the weights w_q, -w_q q (FP32), the integer q vectors, the jump-ahead
@@ -237,6 +240,7 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
     float *wb = reinterpret_cast<float *>(wa + L::WB);
     float *fb = reinterpret_cast<float *>(wa + L::FB);
     float *ub = reinterpret_cast<float *>(wa + L::UB);
+    double *rb = wa + L::RB;
     float *qk = reinterpret_cast<float *>(sm + L::QK);
     int4 *qm = reinterpret_cast<int4 *>(sm + L::QM);
     uint64_t *jmp = reinterpret_cast<uint64_t *>(sm + L::JMP);
@@ -321,8 +325,6 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
     constexpr int kRngBatch = 10;
     static_assert(3 * kRngBatch <= 32 && 5 * kRngBatch == L::NJ, "draw batch");
     const int bm = k / 3, bi = k - 3 * bm;
-    float bgx = 0.0f, bgy = 0.0f;
-    double bu = 0.0;
     float ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};  // mover with partner k (k < NB)
     float pc = 0, ps = 0;  // exp(i q.x_a) of the mover's current position (this lane's q)
     bool have_old = false;
@@ -449,15 +451,27 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
                 float sn, cs;
                 sincos_turns(float(ua - rint(ua)), sn, cs);
                 const float rad = sqrt_r<float>(-2.0f * log_r<float>(float(ubd)));
-                bgx = sn * rad;
-                bgy = cs * rad;
-                bu = ua;  // the acceptance draw in the lanes 3m + 2
+                const float gx = sn * rad, gy = cs * rad;
+                // move m's draws to its 32 B of the walker area
+                float *rf = reinterpret_cast<float *>(rb + 4 * bm);
+                if (on) {
+                    if (bi == 0) {
+                        rf[0] = gx;
+                        rf[1] = gy;
+                    } else if (bi == 1) {
+                        rf[2] = gy;
+                    } else {
+                        rb[4 * bm + 2] = ua;  // the acceptance draw
+                    }
+                }
                 s = half_lane(sa, 3 * cnt - 1, upper);  // the state after the batch
+                wave_sync();
             }
-            g3[0] = half_lane(bgx, 3 * ml, upper);
-            g3[1] = half_lane(bgy, 3 * ml, upper);
-            g3[2] = half_lane(bgy, 3 * ml + 1, upper);
-            u4 = half_lane(bu, 3 * ml + 2, upper);
+            const float4 gq = reinterpret_cast<const float4 *>(rb)[2 * ml];
+            g3[0] = gq.x;
+            g3[1] = gq.y;
+            g3[2] = gq.z;
+            u4 = rb[4 * ml + 2];
         }
         float dro[3], dxm[3];
         double xn[3];
