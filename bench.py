@@ -283,6 +283,7 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
       the FP64 run is nfp64 steps, compared replica by replica with the mixed run's first
       nfp64 steps), and the mixed sweep at tau/2 (nsm steps each, tau/2 after 1000 steps of
       re-equilibration);
+      The single-particle energy at tau = 0 comes from the line through tau and tau/2.
     * the reference's own algorithm, the all-particle step (the device replay that reproduces
       the reference's trajectories), at tau, tau/2, tau/4 and tau/8: nap f steps after 1000 f
       steps (one unit of imaginary time) of re-equilibration (f = 1, 2, 4, 8).  Its time-step
@@ -314,10 +315,12 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
                                          ("allparticle_eighth_tau", "allparticle", DT / 8, 8000,
                                           8 * nap)):
         series[label] = []
-        for g in groups:
+        for r, g in enumerate(groups):
             e = make_engine(mode, nwalk=per, dt=dt, walkers=g, dcut=DCUT_SM)
             series[label].append(_energy_series(e, neq, nmeas))
             e.close()
+            print(f"bench: energy {label} replica {r}: {np.nanmean(series[label][-1]):.5f}",
+                  file=sys.stderr, flush=True)
         runs[label] = dict(_replica_stats([_series_mean(x) for x in series[label]]),
                            steps=nmeas)
         print(f"bench: energy {label}: {runs[label]}", file=sys.stderr, flush=True)
@@ -345,6 +348,8 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
     lsq = _quadratic_extrapolation([DT, DT / 2, DT / 4, DT / 8],
                                    [ev(k)[0] for k in ap_labels], [ev(k)[1] for k in ap_labels])
     sm = ev("mixed_half_tau")
+    # the single-particle sweep's own time-step error, linear at these steps
+    sp0 = (2 * sm[0] - ev("mixed")[0], float(np.hypot(2 * sm[1], ev("mixed")[1])))
     return dict(unit="E_L / (N E_FG)", tau=DT,
                 start=f"{nequil} mixed steps from the reference's initial walkers, then "
                       f"{nrep} independent populations of {per} walkers",
@@ -363,7 +368,9 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
                                                            "through tau .. tau/8"),
                 allparticle_extrapolated_linear=dict(e=ap0_lin[0], err=ap0_lin[1],
                                                      fit="tau/4, tau/8"),
+                singleparticle_extrapolated=dict(e=sp0[0], err=sp0[1], fit="tau, tau/2"),
                 singleparticle_vs_allparticle_extrapolated=diff(sm, ap0),
+                singleparticle_extrapolated_vs_allparticle_extrapolated=diff(sp0, ap0),
                 singleparticle_vs_allparticle_extrapolated_lsq=diff(sm, (lsq["e"], lsq["err"])))
 
 
