@@ -704,6 +704,12 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
     int expo = 0, sgn = 1;
     bool singular = false;
     int permk = 0;  // lane k: the pivot row of step k
+    // the parity of the permutation's inversions, counted as the pivots are chosen: step k's
+    // pivot row p precedes, in the pivot order, exactly the rows still unused below it
+    // (scalar registers: p is wave-uniform)
+    static_assert(NUP <= 64, "rows in a 64-bit mask");
+    uint64_t unused = NUP == 64 ? ~0ull : (1ull << NUP) - 1;
+    int inv_odd = 0;
     // The pivots in blocks of eight, one per tile column BK (a compile-time index, so the
     // pivot column is read and overwritten in place), one pivot per iteration of a rolled
     // loop within a block (a compact body whose registers are those of one step).  Tile row
@@ -769,6 +775,8 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
 #pragma unroll
             for (int b = 0; b < TR; ++b) pv[b] = prow[c + 8 * b];
             permk = tid == k ? p : permk;
+            inv_odd ^= __popcll(unused & ((1ull << p) - 1)) & 1;
+            unused &= ~(1ull << p);
             // elimination of every element, then the pivot column (lanes c == ck, tile
             // column BK) and the pivot row (lanes r == p & 7, tile row ap) overwritten
 #pragma unroll
@@ -802,18 +810,7 @@ __device__ __forceinline__ DetInfo build_inverse(const DevSys &S, const TB &tb, 
     }
     wave_sync();
 
-    // permutation parity: the parity of its inversions (i < j with perm[i] > perm[j]); lane
-    // i counts those of row i, from reads that do not depend on each other (a walk along
-    // the cycles would chain up to n dependent LDS reads)
-    int odd = 0;
-    if (tid < NUP) {
-        const int pi = permk;
-        int cnt = 0;
-#pragma unroll
-        for (int j = 0; j < NUP; ++j) cnt += (j > tid && perm[j] < pi) ? 1 : 0;
-        odd = cnt & 1;
-    }
-    if (__popcll(__ballot(odd)) & 1) sgn = -sgn;
+    if (inv_odd) sgn = -sgn;  // the permutation's parity
     const double logdet = singular ? -1.0e30 : log(prod) + expo * 0.69314718055994530942;
     if (singular) sgn = 1;
 
