@@ -319,7 +319,7 @@ def energy_study(nequil=6000, nsm=4000, nfp64=1000, nap=3000, nrep=4):
             e = make_engine(mode, nwalk=per, dt=dt, walkers=g, dcut=DCUT_SM)
             series[label].append(_energy_series(e, neq, nmeas))
             e.close()
-            print(f"bench: energy {label} replica {r}: {np.nanmean(series[label][-1]):.5f}",
+            print(f"bench: energy {label} replica {r}: {_series_mean(series[label][-1])[0]:.5f}",
                   file=sys.stderr, flush=True)
         runs[label] = dict(_replica_stats([_series_mean(x) for x in series[label]]),
                            steps=nmeas)
@@ -688,13 +688,19 @@ def main():
         print(f"bench: fp64: {fp}", file=sys.stderr, flush=True)
         extra["fp64_gpu_value"] = fp["value"]
         extra["mixed_over_fp64"] = main_run["value"] / fp["value"]
-        extra["precision"] = precision_study()
-        print(f"bench: precision: {extra['precision']}", file=sys.stderr, flush=True)
+        # the studies beside the headline: one that fails is reported in its own field
+        # (with the error), and the line still carries the measured headline
+        def study(key, fn):
+            try:
+                extra[key] = fn()
+            except Exception as ex:  # noqa: BLE001 - reported, not hidden
+                extra[key] = {"failed": f"{type(ex).__name__}: {ex}"}
+            print(f"bench: {key}: {extra[key]}", file=sys.stderr, flush=True)
+
+        study("precision", precision_study)
         if not a.no_energy:
-            extra["dmc_energy"] = energy_study()
-        br = batch_run(2 * a.steps, 3)  # 2000 steps by default, the batch configuration's
-        print(f"bench: batch: {br}", file=sys.stderr, flush=True)
-        extra["batch"] = br
+            study("dmc_energy", energy_study)
+        study("batch", lambda: batch_run(2 * a.steps, 3))  # 2000 steps by default
     sm = a.workload != "allparticle"
     f = flops_per_walker_step(a.workload, part="moves" if sm else "step")
     achieved = f * main_run["pop_mean"] / (main_run["kernel_ms"] * 1e-3) / 1e12
