@@ -93,16 +93,16 @@ struct PairLayout {
     // per walker: positions [NP][3] FP64; the partners' phases [q][ST] complex FP32, which
     // the transposition of the inverse [NUP][TS] FP32 overlays at the switch of spins
     // (reaching into the weights, which are dead there); the mover's weights w_b and the
-    // new pair values f_b (33 FP32 each, in 144 B); w_q exp(i q.x') (32 FP32); the draws of a
-    // batch of 10 moves, 32 B per move (the three gaussians as FP32, the acceptance draw as
-    // FP64)
+    // new pair values f_b (33 FP32 each, in 144 B); w_q exp(i q.x') (32 FP32); the draws of
+    // two batches of 10 moves (the current one and the next), 32 B per move (the three
+    // gaussians as FP32, the acceptance draw as FP64)
     static constexpr int XS = 0;
     static constexpr int PL = XS + 3 * NP;
     static constexpr int WB = PL + NQMAX * ST;
     static constexpr int FB = WB + 18;
     static constexpr int UB = FB + 18;
     static constexpr int RB = UB + 16;
-    static constexpr int WALKER = RB + 4 * 10;
+    static constexpr int WALKER = RB + 2 * 4 * 10;  // two batch buffers
     static_assert(PL + (NUP * TS + 1) / 2 <= WALKER, "transposition area");
     static_assert(PL % 2 == 0 && WB % 2 == 0 && WALKER % 2 == 0, "16-byte alignment");
     // shared: per q the weights w_q, -w_q q (FP32), the integer q vectors, the jump-ahead
@@ -325,10 +325,40 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
     constexpr int kRngBatch = 10;
     static_assert(3 * kRngBatch <= 32 && 5 * kRngBatch == L::NJ, "draw batch");
     const int bm = k / 3, bi = k - 3 * bm;
+    // the draws of cnt (1 .. kRngBatch) moves from the walker state s into batch buffer buf
+    // (move m's 32 B at rb[4 (kRngBatch buf + m)]); s advances to the state after them
+    auto fill_batch = [&](int buf, int cnt) __attribute__((always_inline)) {
+        const bool on = bm < cnt;
+        const bool on1 = on && bi < 2;
+        const int t0 = on ? 5 * bm + 2 * bi : 0, t1 = on1 ? t0 + 1 : 0;  // < NJ
+        const uint64_t ja0 = on ? jmp[t0] : 0, jc0 = on ? jmp[L::NJ + t0] : 0;
+        const uint64_t ja1 = on1 ? jmp[t1] : 0, jc1 = on1 ? jmp[L::NJ + t1] : 0;
+        const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
+        const double ua = double(sa | 1ull) * 0x1p-48, ubd = double(sb | 1ull) * 0x1p-48;
+        float sn, cs;
+        sincos_turns(float(ua - rint(ua)), sn, cs);
+        const float rad = sqrt_r<float>(-2.0f * log_r<float>(float(ubd)));
+        const float gx = sn * rad, gy = cs * rad;
+        double *rm = rb + 4 * (kRngBatch * buf + bm);
+        float *rf = reinterpret_cast<float *>(rm);
+        if (on) {
+            if (bi == 0) {
+                rf[0] = gx;
+                rf[1] = gy;
+            } else if (bi == 1) {
+                rf[2] = gy;
+            } else {
+                rm[2] = ua;  // the acceptance draw
+            }
+        }
+        s = half_lane(sa, 3 * cnt - 1, upper);  // the state after the batch
+        wave_sync();
+    };
     float ob = 0, oj = 0, ou = 0, odx[3] = {0, 0, 0};  // mover with partner k (k < NB)
     float pc = 0, ps = 0;  // exp(i q.x_a) of the mover's current position (this lane's q)
     bool have_old = false;
     wave_sync();
+    if (m1 > m0) fill_batch(0, m1 - m0 < kRngBatch ? m1 - m0 : kRngBatch);  // the first batch
     for (int ip = m0; ip < m1; ++ip) {
         if (ip == NUP && colmode) {
             // down movers from here: rows of the inverse into the lanes through LDS (over the
@@ -439,39 +469,18 @@ __device__ __forceinline__ void sm_moves_pair(const DevSys &S, const MT &mt, dou
         double u4;
         {
             const int ml = (ip - m0) % kRngBatch;  // wave-uniform
-            if (ml == 0) {
-                const int cnt = m1 - ip < kRngBatch ? m1 - ip : kRngBatch;
-                const bool on = bm < cnt;
-                const bool on1 = on && bi < 2;
-                const int t0 = on ? 5 * bm + 2 * bi : 0, t1 = on1 ? t0 + 1 : 0;  // < NJ
-                const uint64_t ja0 = on ? jmp[t0] : 0, jc0 = on ? jmp[L::NJ + t0] : 0;
-                const uint64_t ja1 = on1 ? jmp[t1] : 0, jc1 = on1 ? jmp[L::NJ + t1] : 0;
-                const uint64_t sa = (ja0 * s + jc0) & kMask48, sb = (ja1 * s + jc1) & kMask48;
-                const double ua = double(sa | 1ull) * 0x1p-48, ubd = double(sb | 1ull) * 0x1p-48;
-                float sn, cs;
-                sincos_turns(float(ua - rint(ua)), sn, cs);
-                const float rad = sqrt_r<float>(-2.0f * log_r<float>(float(ubd)));
-                const float gx = sn * rad, gy = cs * rad;
-                // move m's draws to its 32 B of the walker area
-                float *rf = reinterpret_cast<float *>(rb + 4 * bm);
-                if (on) {
-                    if (bi == 0) {
-                        rf[0] = gx;
-                        rf[1] = gy;
-                    } else if (bi == 1) {
-                        rf[2] = gy;
-                    } else {
-                        rb[4 * bm + 2] = ua;  // the acceptance draw
-                    }
-                }
-                s = half_lane(sa, 3 * cnt - 1, upper);  // the state after the batch
-                wave_sync();
-            }
-            const float4 gq = reinterpret_cast<const float4 *>(rb)[2 * ml];
+            const int buf = ((ip - m0) / kRngBatch) & 1;
+            // at a batch's first move, the next batch's draws into the other buffer, a batch
+            // ahead of their use (off the chain of the moves that read them)
+            if (ml == 0 && ip + kRngBatch < m1)
+                fill_batch(buf ^ 1, m1 - ip - kRngBatch < kRngBatch ? m1 - ip - kRngBatch
+                                                                     : kRngBatch);
+            const double *rbb = rb + 4 * kRngBatch * buf;
+            const float4 gq = reinterpret_cast<const float4 *>(rbb)[2 * ml];
             g3[0] = gq.x;
             g3[1] = gq.y;
             g3[2] = gq.z;
-            u4 = rb[4 * ml + 2];
+            u4 = rbb[4 * ml + 2];
         }
         float dro[3], dxm[3];
         double xn[3];
